@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the RBF-FD shallow-water RK4 hot path on one MI355X.
+
+Workload (default): the N=40962 icosahedral mesh, 31-point stencils, FP64,
+one state, RK4 with the fused graph-captured stage kernels.  One "step" is
+one RK4 timestep = 4 eval_RHS passes over every node.  Inputs are generated
+(seeded) in the reference's binary format, prepared by the C host (RCM,
+layout) and resident in HBM before the timed region.
+
+Prints one JSON line:
+  value      eval_RHS throughput, GNode-RHS/s (N * 4 * steps / time)
+  roofline   the dominant kernel (the fused stage kernel) against HBM peak:
+             algorithmic bytes per launch / average launch duration, the
+             duration taken from HIP events recorded on the context's own
+             stream around the timed region
+  cpu_baseline  the reference's own CPU path (oracle/_ref/swe_ref_cfdl.exe,
+             built from the reference sources with -O3 -fopenmp
+             -DRHS_INNER_SIMD) on the same input, timed by its own
+             profiling report, on a bounded sample of steps
+
+Flags: --gpus N (only 1 is supported: every GPU box here has one GPU),
+--steps K, --warmup W, --nodes N, --members B, --exact, --no-cpu.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NNBR = 31
+# Algorithmic HBM bytes per node per fused stage launch, single state:
+# static data 31*(4 B id + 4*8 B weights) + 17*8 B geometry = 1252 B, the
+# node's own state read once (32 B) and the stage epilogue, averaged over the
+# four stages (D,K written; D,H read + D,K written x2; D,H read + H written:
+# (64 + 128 + 128 + 96) / 4 = 104 B).  Per member beyond the first: 32 + 104.
+STATIC_B = NNBR * (4 + 4 * 8) + 17 * 8
+STATE_B = 32 + 104
+
+
+def stage_bytes(n_nodes: int, members: int) -> float:
+    return float(n_nodes) * (STATIC_B + STATE_B * members)
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--nodes", type=int, default=40962)
+    ap.add_argument("--members", type=int, default=1)
+    ap.add_argument("--layout", default="cfdl")
+    ap.add_argument("--exact", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def cpu_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() else n
+
+
+def run_reference_cpu(path: str, n_nodes: int, target_s: float):
+    """Times the reference build's own main() (Eval_Rhs seconds/timestep
+    from its profiling report, attempts >= 2) on a bounded step sample."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "swe_ref_cfdl.exe")
+    if not os.path.exists(exe):
+        return None
+    cores = cpu_cores()
+    env = dict(os.environ, SWE_INPUT_FILE=path, SWE_USE_NETCDF="0", SWE_NUM_COMPUTE_UNITS="1",
+               SWE_USE_RCM="1", SWE_USE_OCL="0", OMP_NUM_THREADS=str(cores),
+               OMP_SCHEDULE="static", OMP_PROC_BIND="close")
+
+    def run(steps, attempts):
+        e = dict(env, SWE_NSTEPS=str(steps), SWE_NATTEMPTS=str(attempts))
+        t0 = time.time()
+        out = subprocess.run([exe], env=e, capture_output=True, text=True, timeout=600).stdout
+        m = re.search(r"Eval_Rhs\s+\(seconds/timestep\) -> \s*Average: \s*([0-9.eE+-]+)", out)
+        return (float(m.group(1)) if m else None), time.time() - t0, out
+    per_step, wall, _ = run(10, 3)
+    if not per_step:
+        return None
+    attempts = 4
+    steps = int(max(10, min(1000, target_s / (attempts * max(per_step, 1e-9)))))
+    per_step, wall, out = run(steps, attempts)
+    if not per_step:
+        return None
+    return {"value": n_nodes * 4 / per_step / 1e9, "unit": "GNode-RHS/s", "cores": cores,
+            "kind": "reference",
+            "sample": (f"reference main() (-O3 -fopenmp -DRHS_INNER_SIMD, FP64, cfdl, RCM) on the "
+                       f"same N={n_nodes} input: {attempts} attempts x {steps} steps, Eval_Rhs "
+                       f"s/step averaged over attempts >= 2 = {per_step:.4e} s "
+                       f"({wall:.1f} s wall)"),
+            "verified": "VERIFICATION" in out}
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this path runs on one GPU "
+              "(each GPU box has one GPU; ensembles shard by member with no exchange)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import swe_pkg
+    pkg = swe_pkg.load()
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.time()
+    path = pkg.mesh.case_path(args.nodes, seed=1)
+    prob = pkg.host.Problem(path, use_rcm=True, layout=args.layout, simd=4)
+    print(f"bench: input ready N={prob.n_nodes} ({time.time() - t0:.1f}s)", file=sys.stderr,
+          flush=True)
+    ctx = pkg.runtime.HipContext(prob, members=args.members, exact=args.exact)
+    for m in range(args.members):
+        ctx.set_state(prob.H_init, m)
+    stream = torch.cuda.ExternalStream(ctx.stream())
+
+    ctx.rk_steps(args.warmup)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    w0 = time.perf_counter()
+    ctx.rk_steps(args.steps)
+    ev1.record(stream)
+    ev1.synchronize()
+    wall = time.perf_counter() - w0
+    torch.cuda.synchronize()
+    gpu_ms = ev0.elapsed_time(ev1)
+    H = ctx.get_state(0)
+    finite = bool(np.isfinite(H).all())
+
+    N = prob.padded
+    launches = 4 * args.steps
+    launch_s = gpu_ms * 1e-3 / launches
+    achieved = stage_bytes(N, args.members) / launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_n{args.nodes}_b{args.members}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fp:
+            traffic = json.load(fp).get("hbm_bytes_per_launch")
+    node_rhs = float(prob.n_nodes) * 4 * args.steps * args.members
+    value = node_rhs / (gpu_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = run_reference_cpu(path, prob.n_nodes, args.cpu_seconds)
+    out = {
+        "metric": "eval_RHS GNodes/s + HBM GB/s; ensemble steps/s and p50/p99 latency",
+        "value": value,
+        "unit": "GNode-RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded icosahedral mesh + Gaussian RBF-FD weights, TC5-like state)",
+        "config": {"workload": f"RBF-FD SWE RK4, N={prob.n_nodes} nodes, stencil 31, "
+                               f"{args.members} member(s), fused stage kernels + HIP graphs",
+                   "nodes": prob.n_nodes, "members": args.members, "layout": args.layout,
+                   "exact": bool(args.exact), "dt": prob.dt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stage_quad" if args.members == 1 else "k_stage_members",
+                     "bytes_per_launch": stage_bytes(N, args.members),
+                     "avg_launch_us": launch_s * 1e6},
+        "cpu_baseline": cpu,
+        "member_steps_per_s": args.members * args.steps / (gpu_ms * 1e-3),
+        "wall_s_timed_region": wall,
+        "state_finite": finite,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,127 @@
+/*
+ * libswe_hip.so -- MI355X (gfx950) HIP implementation of the RBF-FD
+ * shallow-water RK4 hot path, behind a plain C ABI.
+ *
+ * It replaces, as a drop-in, the two compute back-ends the reference driver
+ * selects between at swe_code/main/main.c:338-356:
+ *   - the CPU/OpenMP/OpenACC stage driver
+ *       void RK_substep(fType* H, fType* K, fType* F, fType* D, int substep_id)
+ *       (swe_code/main/include/rk4_rbffd_swe.h:8, rk4_rbffd_swe.c:30-72)
+ *   - the OpenCL path: platform/context/queue set-up and kernel compilation
+ *     (device_setup.c:13-225), load_all_buffers (buffers.c:26), per-attempt
+ *     buffer mapping (main.c:304-322), RK_substep_ocl (RK_ocl.c:9) and the
+ *     read-back map (main.c:379-380).
+ * The static model data are handed over once, in the caller's host layout
+ * (the cfdl/sfdl/SIMD_LENGTH index macros of swe_config.h:21-54), and
+ * repacked on the device; the state H stays on the device between calls.
+ *
+ * Conventions: every function returns 0 (SWE_HIP_OK) or a negative
+ * SWE_HIP_E* code; swe_hip_strerror() names the code and
+ * swe_hip_last_error() gives the detail of the most recent failure on the
+ * calling thread.  The reference's OpenCL helpers print and exit(-1) on any
+ * error (device_setup.c:346); a host that wants that behaviour prints
+ * swe_hip_last_error() and exits.  Host buffers are caller-owned; device
+ * buffers are owned by the context.  One host thread per context; work is
+ * ordered on the context's stream and the functions that return data to the
+ * host synchronise it.  FP64 only.
+ */
+#ifndef SWE_HIP_H
+#define SWE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWE_HIP_OK 0
+#define SWE_HIP_EINVAL (-1)   /* bad argument or unsupported layout         */
+#define SWE_HIP_ENOMEM (-2)   /* device or host allocation failed           */
+#define SWE_HIP_EDEVICE (-3)  /* no HIP device / HIP runtime error          */
+#define SWE_HIP_ERANGE (-4)   /* a stencil index outside the patch          */
+
+/* Context flags. */
+#define SWE_HIP_EXACT 1u      /* sequential stencil sums, no FMA contraction:
+                                 bitwise equal to the reference CPU build
+                                 compiled without contraction              */
+
+/* The static model data of the local patch (PSMD_struct,
+ * swe_code/main/include/swe_config.h:187-233) as plain pointers, in the
+ * host layout described by cfdl/sfdl/simd.  Serial single patch only:
+ * compute_pid_s == 0 and patch_size == compute_size (the whole sphere runs
+ * on one GPU; init_patches.c serial branch). */
+typedef struct swe_hip_static {
+    int compute_size;          /* padded node count                        */
+    int patch_size;
+    int compute_pid_s;
+    int Nnbr;                  /* real stencil width (1..64)               */
+    int padded_Nnbr;           /* row stride of idx/Dx/Dy/Dz/L             */
+    int cfdl, sfdl, simd;      /* USE_CFDL, USE_SFDL, SIMD_LENGTH          */
+    double gh0, dt;
+    const double *x, *y, *z, *f, *ghm;         /* [compute_size]           */
+    const double *p_u, *p_v, *p_w, *gradghm;   /* VECT_LIN_ID layout       */
+    const int *idx;                            /* DM_LIN_ID layout, 0-based*/
+    const double *Dx, *Dy, *Dz, *L;            /* DM_LIN_ID layout, scaled */
+} swe_hip_static;
+
+typedef struct swe_hip_ctx swe_hip_ctx;
+
+/* Per-phase device time accumulated by the drop-in path while timing is on,
+ * seconds (the t_eval_rhs / t_eval_K / t_update_D / t_update_H fields of
+ * timing_struct, swe_config.h:262-279). */
+typedef struct swe_hip_timers {
+    double t_eval_rhs, t_eval_K, t_update_D, t_update_H;
+} swe_hip_timers;
+
+/* Replaces createPlatform/createContext/createCommandQueue/compileKernel/
+ * load_all_buffers (main.c:249-280): validates the static data, repacks it
+ * into the device layout and allocates `members` state slots (1 = the
+ * reference's single state; >1 = an ensemble advanced together).  Every
+ * stencil index is checked against the patch (SWE_HIP_ERANGE). */
+int swe_hip_create(const swe_hip_static *s, int members, unsigned flags,
+                   int device, swe_hip_ctx **out);
+void swe_hip_destroy(swe_hip_ctx *ctx);
+
+/* Per-attempt state upload (main.c:297 + 304-315): H in the host state
+ * layout (SVM_LIN_ID, patch_size*4 values) into member slot `member`. */
+int swe_hip_set_state(swe_hip_ctx *ctx, int member, const double *H);
+/* Read-back (main.c:379-380): member slot -> host state layout. */
+int swe_hip_get_state(swe_hip_ctx *ctx, int member, double *H);
+
+/* Exactly RK_substep / RK_substep_ocl for every member: eval_RHS into F,
+ * then copy/update_D and eval_K/update_H as separate kernels with the
+ * reference's constants (dt/2.0, 2.0, 1.0, dt/6.0).  substep_id 0..3. */
+int swe_hip_rk_substep(swe_hip_ctx *ctx, int substep_id);
+
+/* Fast path: nsteps full RK4 steps (4 fused stage kernels per step, the RHS
+ * never stored, K double-buffered), replayed from captured HIP graphs.
+ * Same arithmetic per node as swe_hip_rk_substep. */
+int swe_hip_rk_steps(swe_hip_ctx *ctx, int nsteps);
+
+/* Test hook: one eval_RHS_swe (rk4_rbffd_swe.c:189-362) of a host state
+ * K into F (host layouts), on member slot `member`. */
+int swe_hip_eval_rhs(swe_hip_ctx *ctx, int member, const double *K, double *F);
+
+/* Timing of the drop-in path (wall clock around each synchronised kernel,
+ * as the OpenCL path's clFinish-per-launch timing, RK_ocl.c:76). */
+int swe_hip_set_timing(swe_hip_ctx *ctx, int on);
+int swe_hip_get_timers(swe_hip_ctx *ctx, swe_hip_timers *t, int reset);
+
+int swe_hip_synchronize(swe_hip_ctx *ctx);
+
+/* Introspection for callers that drive the context's stream themselves
+ * (benchmarks, the ensemble server): device state pointer of the layout
+ * [node][var][member] (double, n_nodes*4*members), the HIP stream as an
+ * opaque handle, and the sizes. */
+int swe_hip_info(const swe_hip_ctx *ctx, int *n_nodes, int *members,
+                 int *nnbr, unsigned *flags);
+void *swe_hip_state_device_ptr(swe_hip_ctx *ctx);
+void *swe_hip_stream(swe_hip_ctx *ctx);
+/* Device bytes held by the context (static data + state buffers). */
+long long swe_hip_device_bytes(const swe_hip_ctx *ctx);
+
+const char *swe_hip_strerror(int code);
+const char *swe_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

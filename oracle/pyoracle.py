@@ -177,3 +177,14 @@ class RefBuild:
         F = np.zeros(4 * self.compute_size)
         self.lib.ref_eval_rhs(_ptr(K), _ptr(F))
         return F
+
+
+def ref_rcm(idx: np.ndarray, layout: str = "cfdl") -> np.ndarray:
+    """The reference's rcm_mapping on a stencil graph (N, W), 0-based ids."""
+    so = os.path.join(HERE, "_ref", f"libswe_ref_{layout}.so")
+    L = C.CDLL(so)
+    L.ref_rcm_of.argtypes = [C.c_int, C.c_int, _ip, _ip]
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    out = np.empty(idx.shape[0], dtype=np.int32)
+    L.ref_rcm_of(idx.shape[0], idx.shape[1], idx.ctypes.data_as(_ip), out.ctypes.data_as(_ip))
+    return out

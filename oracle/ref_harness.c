@@ -138,3 +138,16 @@ void ref_eval_rhs(double *K, double *F)
     local_timer.attempt = 0;
     eval_RHS_swe(K, F);
 }
+
+/* rcm_mapping (rcm.c:9) of an arbitrary stencil graph [N][W] (0-based). */
+void ref_rcm_of(int N, int W, const int *idx, int *out)
+{
+    GSMD_struct G;
+    memset(&G, 0, sizeof(G));
+    G.Nnodes = N;
+    G.Nnbr = W;
+    G.idx = (int *)idx;
+    int *m = rcm_mapping(G);
+    memcpy(out, m, sizeof(int) * N);
+    free(m);
+}

@@ -1,0 +1,357 @@
+// Device kernels of the RBF-FD shallow-water RK4 path for gfx950 (CDNA4).
+//
+// One RK stage = one launch: the 31-point stencil sums of Dx, Dy, Dz and the
+// hyperviscosity operator L over the neighbour states, the pointwise SWE
+// right-hand side (advection, Coriolis, pressure gradient, tangent
+// projection, orography, continuity; swe_code/main/rk4_rbffd_swe.c:285-353)
+// and the RK4 stage update of that node (rk4_rbffd_swe.c:30-186) fused into
+// the epilogue, so the RHS is never written to memory.
+//
+// Device layouts (HBM):
+//   nbr  int32  [nnbr][ld]        neighbour ids, stencil-slot major
+//   wts  f64    [nnbr][ld][4]     (Dx, Dy, Dz, L) of one (slot, node): 32 B
+//   geo  f64    [GEO_N][ld]       x y z f ghm p_u[3] p_v[3] p_w[3] gradghm[3]
+//   state f64   [node][var][B]    var = u v w h, member fastest
+// Every wave reads contiguous runs of the weight and index streams; the
+// state gathers hit L2 because nodes are RCM-ordered and each XCD owns one
+// contiguous node range (xcd_block).
+//
+// The build uses -ffp-contract=off.  EXACT kernels therefore round every
+// product and sum separately, in the reference's order (sequential stencil
+// slot j, rk4_rbffd_swe.c:285-316, expressions of :331-353, axpys of
+// :108-183), and reproduce the reference CPU build bit for bit; the fast
+// kernels use fused multiply-adds for the stencil sums and stage updates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace swe {
+
+constexpr int GEO_N = 17;
+enum GeoField {
+    G_X = 0, G_Y, G_Z, G_F, G_GHM,
+    G_PU = 5, G_PV = 8, G_PW = 11, G_GG = 14
+};
+
+// Kernel variants: the four RK stages and a plain RHS evaluation.
+enum StageKind { ST_0 = 0, ST_1 = 1, ST_2 = 2, ST_3 = 3, ST_RHS = 4 };
+
+struct DevMesh {
+    const int *__restrict__ nbr;
+    const double *__restrict__ wts;
+    const double *__restrict__ geo;
+    int n;        // nodes
+    int ld;       // leading dimension of nbr/wts/geo (>= n)
+    int nnbr;     // stencil width (loop bound)
+    double gh0;
+};
+
+struct DevStage {
+    const double *src;  // gather source: H (stage 0) or K
+    const double *h;    // H, own node (stage update)
+    double *d;          // D, running RK sum
+    double *out;        // next K, H (stage 3) or F (ST_RHS)
+    double c;           // K = H + c*F (dt/2, dt/2, dt); stage 3: H += c*D (dt/6)
+    double dcoef;       // D += dcoef*F (2, 2, 1)
+    int members;        // B: state stride of a (node, var) row
+    int nblk;           // grid.x, a multiple of 8
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remap the
+// hardware block id so that XCD x processes the contiguous logical blocks
+// [x*nblk/8, (x+1)*nblk/8): its L2 then holds the state of one node band.
+__device__ __forceinline__ int xcd_block(int nblk)
+{
+    const int b = blockIdx.x;
+    return (b & 7) * (nblk >> 3) + (b >> 3);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ double madd(double acc, double w, double s)
+{
+    if constexpr (EXACT) {
+        const double p = w * s;
+        return acc + p;
+    } else {
+        return __builtin_fma(w, s, acc);
+    }
+}
+
+// y + (a*x), the reference's axpy form.
+template <bool EXACT>
+__device__ __forceinline__ double axpy(double y, double a, double x)
+{
+    if constexpr (EXACT) {
+        const double p = a * x;
+        return y + p;
+    } else {
+        return __builtin_fma(a, x, y);
+    }
+}
+
+struct Sums {
+    double dudx, dvdx, dwdx, dhdx;
+    double dudy, dvdy, dwdy, dhdy;
+    double dudz, dvdz, dwdz, dhdz;
+    double lu, lv, lw, lh;
+};
+
+struct Geo {
+    double x, y, z, f, ghm;
+    double pu0, pu1, pu2, pv0, pv1, pv2, pw0, pw1, pw2, gx, gy, gz;
+};
+
+__device__ __forceinline__ Geo load_geo(const double *__restrict__ geo, int ld, int i)
+{
+    Geo g;
+    g.x = geo[G_X * (long)ld + i];
+    g.y = geo[G_Y * (long)ld + i];
+    g.z = geo[G_Z * (long)ld + i];
+    g.f = geo[G_F * (long)ld + i];
+    g.ghm = geo[G_GHM * (long)ld + i];
+    g.pu0 = geo[(G_PU + 0) * (long)ld + i];
+    g.pu1 = geo[(G_PU + 1) * (long)ld + i];
+    g.pu2 = geo[(G_PU + 2) * (long)ld + i];
+    g.pv0 = geo[(G_PV + 0) * (long)ld + i];
+    g.pv1 = geo[(G_PV + 1) * (long)ld + i];
+    g.pv2 = geo[(G_PV + 2) * (long)ld + i];
+    g.pw0 = geo[(G_PW + 0) * (long)ld + i];
+    g.pw1 = geo[(G_PW + 1) * (long)ld + i];
+    g.pw2 = geo[(G_PW + 2) * (long)ld + i];
+    g.gx = geo[(G_GG + 0) * (long)ld + i];
+    g.gy = geo[(G_GG + 1) * (long)ld + i];
+    g.gz = geo[(G_GG + 2) * (long)ld + i];
+    return g;
+}
+
+// Pointwise right-hand side of one node, operation order of
+// rk4_rbffd_swe.c:331-353 (no contraction in this build).
+__device__ __forceinline__ void rhs_point(const Sums &S, double u, double v, double w,
+                                          double h, const Geo &g, double gh0, double F[4])
+{
+    const double ru = -((u * S.dudx) + (v * S.dudy) + (w * S.dudz)
+                        + (g.f * ((g.y * w) - (g.z * v))) + S.dhdx);
+    const double rv = -((u * S.dvdx) + (v * S.dvdy) + (w * S.dvdz)
+                        + (g.f * ((g.z * u) - (g.x * w))) + S.dhdy);
+    const double rw = -((u * S.dwdx) + (v * S.dwdy) + (w * S.dwdz)
+                        + (g.f * ((g.x * v) - (g.y * u))) + S.dhdz);
+    F[0] = (g.pu0 * ru) + (g.pu1 * rv) + (g.pu2 * rw) + S.lu;
+    F[1] = (g.pv0 * ru) + (g.pv1 * rv) + (g.pv2 * rw) + S.lv;
+    F[2] = (g.pw0 * ru) + (g.pw1 * rv) + (g.pw2 * rw) + S.lw;
+    F[3] = -((u * (S.dhdx - g.gx)) + (v * (S.dhdy - g.gy)) + (w * (S.dhdz - g.gz))
+             + ((h + gh0 - g.ghm) * (S.dudx + S.dvdy + S.dwdz)))
+           + S.lh;
+}
+
+// Stage update of one state value at linear index e (RK_substep,
+// rk4_rbffd_swe.c:36-69):
+//   stage 0: D = F;             K = H + (dt/2)*F
+//   stage 1: D = D + 2*F;       K = H + (dt/2)*F
+//   stage 2: D = D + 2*F;       K = H + dt*F
+//   stage 3: D = D + 1*F;       H = H + (dt/6)*D   (D is dead afterwards)
+template <int STAGE, bool EXACT>
+__device__ __forceinline__ void stage_update(const DevStage &st, long e, double Fv)
+{
+    if constexpr (STAGE == ST_RHS) {
+        st.out[e] = Fv;
+    } else if constexpr (STAGE == ST_0) {
+        const double hv = st.h[e];
+        st.d[e] = Fv;
+        st.out[e] = axpy<EXACT>(hv, st.c, Fv);
+    } else if constexpr (STAGE == ST_1 || STAGE == ST_2) {
+        const double hv = st.h[e];
+        const double dv = axpy<EXACT>(st.d[e], st.dcoef, Fv);
+        st.d[e] = dv;
+        st.out[e] = axpy<EXACT>(hv, st.c, Fv);
+    } else {
+        const double hv = st.h[e];
+        const double dv = axpy<EXACT>(st.d[e], st.dcoef, Fv);
+        st.out[e] = axpy<EXACT>(hv, st.c, dv);
+    }
+}
+
+// Broadcast lane K of every aligned group of 4 lanes (DPP quad_perm).
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v)
+{
+    constexpr int ctrl = K * 0x55;  // quad_perm [K, K, K, K]
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// ---------------------------------------------------------------------------
+// Single state (B == 1): four lanes per node.  Lane k of the quad reads
+// variable k of every neighbour and keeps the Dx, Dy, Dz, L sums of that
+// variable (each a sequential sum over the stencil); the 16 sums are then
+// shared across the quad by DPP, every lane evaluates the pointwise RHS and
+// lane k updates variable k.  256 threads = 64 nodes per workgroup.
+template <int STAGE, bool EXACT, int NNBR>
+__global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
+{
+    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int node_raw = t >> 2;
+    const int k = t & 3;
+    const bool live = node_raw < m.n;
+    const int node = live ? node_raw : m.n - 1;  // keep quads whole for DPP
+    const int nn = NNBR > 0 ? NNBR : m.nnbr;
+    const long ld = m.ld;
+    const int *__restrict__ nb = m.nbr + node;
+    const double *__restrict__ wp = m.wts + 4L * node;
+    const double *__restrict__ src = st.src;
+
+    double sx = 0.0, sy = 0.0, sz = 0.0, sl = 0.0;
+    auto slot = [&](int j) {
+        const int q = nb[j * ld];
+        const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+        const double s = src[4L * q + k];
+        sx = madd<EXACT>(sx, w01.x, s);
+        sy = madd<EXACT>(sy, w01.y, s);
+        sz = madd<EXACT>(sz, w23.x, s);
+        sl = madd<EXACT>(sl, w23.y, s);
+    };
+    if constexpr (NNBR > 0) {
+#pragma unroll
+        for (int j = 0; j < NNBR; ++j)
+            slot(j);
+    } else {
+#pragma unroll 4
+        for (int j = 0; j < nn; ++j)
+            slot(j);
+    }
+    Sums S;
+    S.dudx = quad_bcast<0>(sx); S.dudy = quad_bcast<0>(sy);
+    S.dudz = quad_bcast<0>(sz); S.lu = quad_bcast<0>(sl);
+    S.dvdx = quad_bcast<1>(sx); S.dvdy = quad_bcast<1>(sy);
+    S.dvdz = quad_bcast<1>(sz); S.lv = quad_bcast<1>(sl);
+    S.dwdx = quad_bcast<2>(sx); S.dwdy = quad_bcast<2>(sy);
+    S.dwdz = quad_bcast<2>(sz); S.lw = quad_bcast<2>(sl);
+    S.dhdx = quad_bcast<3>(sx); S.dhdy = quad_bcast<3>(sy);
+    S.dhdz = quad_bcast<3>(sz); S.lh = quad_bcast<3>(sl);
+
+    const double2 s01 = *reinterpret_cast<const double2 *>(src + 4L * node);
+    const double2 s23 = *reinterpret_cast<const double2 *>(src + 4L * node + 2);
+    const Geo g = load_geo(m.geo, m.ld, node);
+    double F[4];
+    rhs_point(S, s01.x, s01.y, s23.x, s23.y, g, m.gh0, F);
+    const double Fk = k == 0 ? F[0] : k == 1 ? F[1] : k == 2 ? F[2] : F[3];
+    if (live)
+        stage_update<STAGE, EXACT>(st, 4L * node + k, Fk);
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble (B >= 2): lane = (node, member), MB members of one node per lane
+// group, member block blockIdx.y.  Each lane keeps all 16 sums of its
+// member; the stencil weights and ids are shared by the MB lanes of a node
+// (wave-uniform scalar loads when MB == 64), so one weight read from HBM
+// serves every member of the block.  The neighbour state of a variable is a
+// contiguous run of MB doubles.
+template <int STAGE, bool EXACT, int MB, int NNBR>
+__global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
+{
+    static_assert(MB >= 2 && MB <= 64 && (MB & (MB - 1)) == 0, "MB power of two");
+    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    int node_raw = t / MB;
+    if constexpr (MB == 64)
+        node_raw = __builtin_amdgcn_readfirstlane(node_raw);
+    const int mem = (int)blockIdx.y * MB + (t & (MB - 1));
+    const bool live = node_raw < m.n;
+    const int node = live ? node_raw : m.n - 1;
+    const int nn = NNBR > 0 ? NNBR : m.nnbr;
+    const long ld = m.ld;
+    const long B = st.members;
+    const int *__restrict__ nb = m.nbr + node;
+    const double *__restrict__ wp = m.wts + 4L * node;
+    const double *__restrict__ src = st.src + mem;
+
+    Sums S = {};
+    auto slot = [&](int j) {
+        int q = nb[j * ld];
+        if constexpr (MB == 64)
+            q = __builtin_amdgcn_readfirstlane(q);
+        const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+        const double *sq = src + 4L * q * B;
+        const double su = sq[0], sv = sq[B], sw = sq[2 * B], sh = sq[3 * B];
+        S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
+        S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
+        S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
+        S.dwdy = madd<EXACT>(S.dwdy, w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, w01.y, sh);
+        S.dudz = madd<EXACT>(S.dudz, w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, w23.x, sv);
+        S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
+        S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
+        S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
+    };
+    if constexpr (NNBR > 0) {
+#pragma unroll
+        for (int j = 0; j < NNBR; ++j)
+            slot(j);
+    } else {
+#pragma unroll 4
+        for (int j = 0; j < nn; ++j)
+            slot(j);
+    }
+    const double *own = src + 4L * node * B;
+    const double u = own[0], v = own[B], w = own[2 * B], h = own[3 * B];
+    const Geo g = load_geo(m.geo, m.ld, node);
+    double F[4];
+    rhs_point(S, u, v, w, h, g, m.gh0, F);
+    if (live) {
+        const long e = 4L * node * B + mem;
+#pragma unroll
+        for (int var = 0; var < 4; ++var)
+            stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise kernels of the drop-in stage driver (rk4_rbffd_swe.c:74-186),
+// over `count` contiguous values.
+__global__ __launch_bounds__(256) void k_copy(double *__restrict__ dst,
+                                              const double *__restrict__ src, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        dst[i] = src[i];
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_axpy_inplace(double *__restrict__ y, double a,
+                                                      const double *__restrict__ x, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        y[i] = axpy<EXACT>(y[i], a, x[i]);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_axpy_to(double *__restrict__ out,
+                                                 const double *__restrict__ y, double a,
+                                                 const double *__restrict__ x, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        out[i] = axpy<EXACT>(y[i], a, x[i]);
+}
+
+// Member slot <-> node-major [node][var] staging copy.
+__global__ __launch_bounds__(256) void k_put_member(double *__restrict__ state,
+                                                    const double *__restrict__ flat,
+                                                    int member, int B, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        state[i * B + member] = flat[i];
+}
+
+__global__ __launch_bounds__(256) void k_get_member(double *__restrict__ flat,
+                                                    const double *__restrict__ state,
+                                                    int member, int B, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        flat[i] = state[i * B + member];
+}
+
+}  // namespace swe

@@ -1,0 +1,161 @@
+/*
+ * swe_run: the reference driver (swe_code/main/main.c) on the HIP path.
+ *
+ * Same run-time parameters (swe_code/main/runtime_params.c:12-40):
+ *   SWE_INPUT_FILE, SWE_USE_NETCDF, SWE_NUM_COMPUTE_UNITS, SWE_NSTEPS,
+ *   SWE_NATTEMPTS, SWE_USE_RCM, SWE_USE_OCL (accepted; the compute back-end
+ *   is always libswe_hip)
+ * and run-time replacements of the reference's compile-time switches:
+ *   SWE_LAYOUT       default | cfdl | sfdl | cfdl_sfdl   (USE_CFDL/USE_SFDL)
+ *   SWE_SIMD_LENGTH  SIMD_LENGTH (default 4)
+ *   SWE_HIP_MODE     substep (RK_substep drop-in, 4 calls per step, as
+ *                    main.c:336-358) | steps (fused, graph-captured)
+ *   SWE_HIP_EXACT    1 = bitwise reference arithmetic (default 0)
+ *   SWE_MEMBERS      ensemble members advanced together (default 1)
+ * Init pipeline main.c:94-190, attempt loop :294-392, verify_output
+ * :439-477 (every member against H_100), profiling report profiling.c.
+ * Errors: a missing parameter or a NetCDF request prints and exits 1, an
+ * unreadable input exits 2 (input.c CHECK_OPEN_ERR), a HIP error prints
+ * the library's message and exits -1 (device_setup.c checkError).
+ */
+#include "swe_hip.h"
+#include "swe_host.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const char *need_env(const char *name)
+{
+    const char *v = getenv(name);
+    if (!v) {
+        fprintf(stderr, "swe_run: environment variable %s is not set\n", name);
+        exit(1);
+    }
+    return v;
+}
+
+static int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
+static void hip_check(int rc, const char *what)
+{
+    if (rc != SWE_HIP_OK) {
+        printf("Error in %s: %s: %s\n", what, swe_hip_strerror(rc), swe_hip_last_error());
+        exit(-1);
+    }
+}
+
+int main(void)
+{
+    const double t_start = swe_get_time();
+    const char *input = need_env("SWE_INPUT_FILE");
+    const int use_netcdf = atoi(need_env("SWE_USE_NETCDF"));
+    (void)atoi(need_env("SWE_NUM_COMPUTE_UNITS"));
+    const int nsteps = atoi(need_env("SWE_NSTEPS"));
+    const int nattempts = atoi(need_env("SWE_NATTEMPTS"));
+    const int use_rcm = atoi(need_env("SWE_USE_RCM"));
+    const char *layout_name = getenv("SWE_LAYOUT") ? getenv("SWE_LAYOUT") : "default";
+    const char *mode = getenv("SWE_HIP_MODE") ? getenv("SWE_HIP_MODE") : "substep";
+    const int exact = env_int("SWE_HIP_EXACT", 0);
+    const int members = env_int("SWE_MEMBERS", 1);
+    swe_layout lay = {0, 0, env_int("SWE_SIMD_LENGTH", 4)};
+    if (!strcmp(layout_name, "cfdl")) lay.cfdl = 1;
+    else if (!strcmp(layout_name, "sfdl")) lay.sfdl = 1;
+    else if (!strcmp(layout_name, "cfdl_sfdl")) lay.cfdl = lay.sfdl = 1;
+    else if (strcmp(layout_name, "default")) {
+        fprintf(stderr, "swe_run: unknown SWE_LAYOUT %s\n", layout_name);
+        return 1;
+    }
+    if (use_netcdf == 1) {
+        printf("Executable was not built for NetCDF I/O compatibility.\n\n");
+        return 1;
+    }
+    if (nsteps < 0 || nattempts < 1 || members < 1) {
+        fprintf(stderr, "swe_run: bad SWE_NSTEPS/SWE_NATTEMPTS/SWE_MEMBERS\n");
+        return 1;
+    }
+    printf("\n\n============================= Runtime Configurations =============================\n\n");
+    printf("Input from file: yes\n\tInput File Path: %s\n\n", input);
+    printf("Simulation length (timesteps): %d\n", nsteps);
+    printf("Number of attempts: %d\n", nattempts);
+    printf("USE RCM Node Ordering: %s\n", use_rcm == 1 ? "yes" : "no");
+    printf("Layout: %s (SIMD_LENGTH %d), HIP mode: %s%s, members: %d\n", layout_name, lay.simd,
+           mode, exact ? " (exact)" : "", members);
+
+    swe_timers timer;
+    if (swe_timers_init(&timer, nattempts)) {
+        fprintf(stderr, "swe_run: timer allocation failed\n");
+        return 1;
+    }
+    swe_problem prob;
+    int err = swe_problem_load(input, use_rcm == 1, &lay, &prob);
+    if (err == SWE_HOST_EOPEN) {
+        printf("Error opening file %s\n", input);
+        return 2;
+    }
+    if (err) {
+        fprintf(stderr, "swe_run: cannot prepare %s (host error %d)\n", input, err);
+        return 1;
+    }
+    timer.t_init = swe_get_time() - t_start;
+
+    const swe_patch *p = &prob.patch;
+    swe_hip_static st = {p->compute_size, p->patch_size, p->compute_pid_s, p->Nnbr,
+                         p->padded_Nnbr, lay.cfdl, lay.sfdl, lay.simd, p->gh0, p->dt,
+                         p->x, p->y, p->z, p->f, p->ghm, p->p_u, p->p_v, p->p_w,
+                         p->gradghm, p->idx, p->Dx, p->Dy, p->Dz, p->L};
+    const double t_dev = swe_get_time();
+    swe_hip_ctx *ctx = NULL;
+    hip_check(swe_hip_create(&st, members, exact ? SWE_HIP_EXACT : 0u, 0, &ctx), "swe_hip_create");
+    timer.t_dev = swe_get_time() - t_dev;
+    const int substep_mode = strcmp(mode, "steps") != 0;
+    double *H = (double *)malloc(sizeof(double) * 4 * (size_t)p->patch_size);
+    if (!H)
+        return 1;
+    for (int attempt = 0; attempt < nattempts; attempt++) {
+        for (int m = 0; m < members; m++)
+            hip_check(swe_hip_set_state(ctx, m, prob.H_init), "swe_hip_set_state");
+        hip_check(swe_hip_set_timing(ctx, substep_mode), "swe_hip_set_timing");
+        timer.attempt = attempt;
+        const double t0 = swe_get_time();
+        if (substep_mode) {
+            for (int i = 1; i <= nsteps; i++)
+                for (int rk = 0; rk < 4; rk++)
+                    hip_check(swe_hip_rk_substep(ctx, rk), "swe_hip_rk_substep");
+        } else {
+            hip_check(swe_hip_rk_steps(ctx, nsteps), "swe_hip_rk_steps");
+        }
+        hip_check(swe_hip_synchronize(ctx), "swe_hip_synchronize");
+        timer.t_main[attempt] = swe_get_time() - t0;
+        swe_hip_timers ht;
+        hip_check(swe_hip_get_timers(ctx, &ht, 1), "swe_hip_get_timers");
+        timer.t_eval_rhs[attempt] = ht.t_eval_rhs;
+        timer.t_eval_K[attempt] = ht.t_eval_K;
+        timer.t_update_D[attempt] = ht.t_update_D;
+        timer.t_update_H[attempt] = ht.t_update_H;
+        int worst = 0;
+        for (int m = 0; m < members; m++) {
+            hip_check(swe_hip_get_state(ctx, m, H), "swe_hip_get_state");
+            const int c = swe_verify_output(p, &lay, H, prob.H_100, 1.0e-9);
+            if (c > worst)
+                worst = c;
+        }
+        printf("Attempt %d:\t", attempt + 1);
+        if (worst != 0)
+            printf("\n\nVERIFICATION FAILED: There were a total of %d values that exceeded an error of %.1e\n\n",
+                   worst, 1.0e-9);
+        else
+            printf("VERIFICATION SUCCEEDED\n");
+        fflush(stdout);
+    }
+    swe_process_profiling(&timer, nsteps, 1, NULL);
+    free(H);
+    swe_hip_destroy(ctx);
+    swe_problem_free(&prob);
+    swe_timers_free(&timer);
+    return 0;
+}

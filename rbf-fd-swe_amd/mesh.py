@@ -157,10 +157,32 @@ def default_dt(n_nodes: int) -> float:
     return 900.0 * 2.0 ** (4 - level_of(n_nodes))
 
 
-# Hyperviscosity coefficient gamma = -GAMMA_C * N**-4 (the shipped 2562 file
-# has GAMMA_C = 0.05; the damping rate of the grid-scale modes of gamma*L is
-# then independent of N).
-DEFAULT_GAMMA_C = 0.5
+# Hyperviscosity strength: gamma is set so that dt * rho(gamma * L) = KAPPA,
+# where rho is the spectral radius of the assembled L (the grid-scale decay
+# rate).  The shipped 2562 file has gamma = -0.05 N**-4, i.e. dt*rho ~ 0.014,
+# too weak to hold its run past ~300 steps; with the shape parameter above,
+# rho(L) does not scale exactly as N**4, so a fixed gamma*N**4 is unstable
+# at some N.  RK4's stability interval on the negative real axis is 2.785.
+DEFAULT_KAPPA = 1.5
+
+
+def spectral_radius(W: np.ndarray, idx: np.ndarray, iters: int = 300) -> float:
+    """Estimate of the largest |eigenvalue| of the sparse matrix with rows W
+    at columns idx: geometric-mean growth of |M^k v| over the last third of
+    `iters` power iterations from a seeded start vector."""
+    import scipy.sparse as sp
+    N, n = idx.shape
+    M = sp.csr_matrix((W.ravel(), (np.repeat(np.arange(N), n), idx.ravel())), shape=(N, N))
+    v = np.random.default_rng(12345).standard_normal(N)
+    v /= np.linalg.norm(v)
+    logs = []
+    for _ in range(iters):
+        w = M @ v
+        nrm = float(np.linalg.norm(w))
+        logs.append(math.log(nrm))
+        v = w / nrm
+    tail = logs[2 * iters // 3:]
+    return math.exp(sum(tail) / len(tail))
 
 
 @dataclass
@@ -207,7 +229,7 @@ def cosine_bell(X: np.ndarray, lat0: float, lon0: float, height: float,
 
 
 def make_case(n_nodes: int, seed: int = 1, dt: float | None = None,
-              gamma_c: float = DEFAULT_GAMMA_C, nnbr: int = NNBR) -> SweCase:
+              kappa: float = DEFAULT_KAPPA, nnbr: int = NNBR) -> SweCase:
     rng = np.random.default_rng(seed)
     u0 = float(rng.uniform(15.0, 25.0))
     depth = float(rng.uniform(5000.0, 6000.0))
@@ -225,9 +247,10 @@ def make_case(n_nodes: int, seed: int = 1, dt: float | None = None,
     H0 = tc5_state(X, u0)
     if dt is None:
         dt = default_dt(n_nodes)
-    gamma = -gamma_c * float(n_nodes) ** -4
+    rho = spectral_radius(L, idx)
+    gamma = -kappa / (dt * rho)
     params = dict(seed=seed, u0=u0, depth=depth, hill=hill, lat0=lat0, lon0=lon0,
-                  eps=eps, dt=dt, gamma_c=gamma_c)
+                  eps=eps, dt=dt, kappa=kappa, rho_L=rho)
     return SweCase(n_nodes=n_nodes, nnbr=nnbr, a=A_EARTH, gh0=GRAV * depth, dt=float(dt),
                    gamma=gamma, X=X, f=2.0 * OMEGA * X[:, 2], ghm=ghm, p_u=p_u, p_v=p_v,
                    p_w=p_w, gradghm=gradghm, Dx=Dx, Dy=Dy, Dz=Dz, L=L, idx=idx,
@@ -305,12 +328,12 @@ def cache_dir() -> str:
 
 
 def case_path(n_nodes: int, seed: int = 1, dt: float | None = None,
-              gamma_c: float = DEFAULT_GAMMA_C) -> str:
+              kappa: float = DEFAULT_KAPPA) -> str:
     """Generate (once, cached on disk) and return the input file path."""
     if dt is None:
         dt = default_dt(n_nodes)
-    tag = hashlib.sha1(f"v1:{n_nodes}:{seed}:{dt!r}:{gamma_c!r}".encode()).hexdigest()[:12]
+    tag = hashlib.sha1(f"v2:{n_nodes}:{seed}:{dt!r}:{kappa!r}".encode()).hexdigest()[:12]
     path = os.path.join(cache_dir(), f"icos{n_nodes}_s{seed}_{tag}.bin")
     if not os.path.exists(path):
-        write_bin(path, make_case(n_nodes, seed=seed, dt=dt, gamma_c=gamma_c))
+        write_bin(path, make_case(n_nodes, seed=seed, dt=dt, kappa=kappa))
     return path

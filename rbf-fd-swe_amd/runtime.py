@@ -1,0 +1,203 @@
+"""ctypes binding of libswe_hip.so (include/swe_hip.h).
+
+There is no CPU fallback: if the library or a HIP device is missing, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+SWE_HIP_EXACT = 1
+
+# Every function include/swe_hip.h declares.
+EXPORTS = (
+    "swe_hip_create", "swe_hip_destroy", "swe_hip_set_state", "swe_hip_get_state",
+    "swe_hip_rk_substep", "swe_hip_rk_steps", "swe_hip_eval_rhs", "swe_hip_set_timing",
+    "swe_hip_get_timers", "swe_hip_synchronize", "swe_hip_info",
+    "swe_hip_state_device_ptr", "swe_hip_stream", "swe_hip_device_bytes",
+    "swe_hip_strerror", "swe_hip_last_error",
+)
+
+
+class SweHipStatic(C.Structure):
+    _fields_ = [("compute_size", C.c_int), ("patch_size", C.c_int),
+                ("compute_pid_s", C.c_int), ("Nnbr", C.c_int), ("padded_Nnbr", C.c_int),
+                ("cfdl", C.c_int), ("sfdl", C.c_int), ("simd", C.c_int),
+                ("gh0", C.c_double), ("dt", C.c_double),
+                ("x", _dp), ("y", _dp), ("z", _dp), ("f", _dp), ("ghm", _dp),
+                ("p_u", _dp), ("p_v", _dp), ("p_w", _dp), ("gradghm", _dp),
+                ("idx", _ip), ("Dx", _dp), ("Dy", _dp), ("Dz", _dp), ("L", _dp)]
+
+
+class SweHipTimers(C.Structure):
+    _fields_ = [("t_eval_rhs", C.c_double), ("t_eval_K", C.c_double),
+                ("t_update_D", C.c_double), ("t_update_H", C.c_double)]
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "libswe_hip.so")
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.swe_hip_create.argtypes = [C.POINTER(SweHipStatic), C.c_int, C.c_uint, C.c_int,
+                                     C.POINTER(vp)]
+        L.swe_hip_destroy.argtypes = [vp]
+        L.swe_hip_destroy.restype = None
+        L.swe_hip_set_state.argtypes = [vp, C.c_int, _dp]
+        L.swe_hip_get_state.argtypes = [vp, C.c_int, _dp]
+        L.swe_hip_rk_substep.argtypes = [vp, C.c_int]
+        L.swe_hip_rk_steps.argtypes = [vp, C.c_int]
+        L.swe_hip_eval_rhs.argtypes = [vp, C.c_int, _dp, _dp]
+        L.swe_hip_set_timing.argtypes = [vp, C.c_int]
+        L.swe_hip_get_timers.argtypes = [vp, C.POINTER(SweHipTimers), C.c_int]
+        L.swe_hip_synchronize.argtypes = [vp]
+        L.swe_hip_info.argtypes = [vp, _ip, _ip, _ip, C.POINTER(C.c_uint)]
+        L.swe_hip_state_device_ptr.argtypes = [vp]
+        L.swe_hip_state_device_ptr.restype = vp
+        L.swe_hip_stream.argtypes = [vp]
+        L.swe_hip_stream.restype = vp
+        L.swe_hip_device_bytes.argtypes = [vp]
+        L.swe_hip_device_bytes.restype = C.c_longlong
+        L.swe_hip_strerror.argtypes = [C.c_int]
+        L.swe_hip_strerror.restype = C.c_char_p
+        L.swe_hip_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class SweHipError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        L = lib()
+        msg = L.swe_hip_strerror(code).decode()
+        detail = L.swe_hip_last_error().decode()
+        super().__init__(f"{where}: {msg} ({code}): {detail}")
+        self.code = code
+
+
+def _chk(rc: int, where: str) -> None:
+    if rc != 0:
+        raise SweHipError(rc, where)
+
+
+def static_from_problem(prob) -> tuple[SweHipStatic, list]:
+    """swe_hip_static over the arrays of a host.Problem (kept alive by the
+    returned list)."""
+    keep = []
+
+    def p(a, dtype=np.float64, t=_dp):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data_as(t)
+    s = SweHipStatic()
+    s.compute_size, s.patch_size = prob.compute_size, prob.patch_size
+    s.compute_pid_s = prob.compute_pid_s
+    s.Nnbr, s.padded_Nnbr = prob.nnbr, prob.padded_nnbr
+    s.cfdl, s.sfdl, s.simd = prob.layout.cfdl, prob.layout.sfdl, prob.layout.simd
+    s.gh0, s.dt = prob.gh0, prob.dt
+    for nm in ("x", "y", "z", "f", "ghm", "p_u", "p_v", "p_w", "gradghm", "Dx", "Dy", "Dz", "L"):
+        setattr(s, nm, p(getattr(prob, nm)))
+    s.idx = p(prob.idx, np.int32, _ip)
+    return s, keep
+
+
+class HipContext:
+    """One device context: static data of one mesh + `members` state slots."""
+
+    def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0):
+        self.prob = prob
+        self.members = int(members)
+        self.exact = bool(exact)
+        s, keep = static_from_problem(prob)
+        h = C.c_void_p()
+        _chk(lib().swe_hip_create(C.byref(s), self.members,
+                                  SWE_HIP_EXACT if exact else 0, device, C.byref(h)),
+             "swe_hip_create")
+        self._h = h
+        self.n_state = 4 * prob.patch_size
+
+    @property
+    def handle(self) -> C.c_void_p:
+        if self._h is None:
+            raise RuntimeError("context is closed")
+        return self._h
+
+    def set_state(self, H: np.ndarray, member: int = 0) -> None:
+        H = np.ascontiguousarray(H, dtype=np.float64)
+        if H.size != self.n_state:
+            raise ValueError(f"state has {H.size} values, expected {self.n_state}")
+        _chk(lib().swe_hip_set_state(self.handle, member, H.ctypes.data_as(_dp)),
+             "swe_hip_set_state")
+
+    def get_state(self, member: int = 0) -> np.ndarray:
+        H = np.empty(self.n_state)
+        _chk(lib().swe_hip_get_state(self.handle, member, H.ctypes.data_as(_dp)),
+             "swe_hip_get_state")
+        return H
+
+    def rk_substep(self, s: int) -> None:
+        _chk(lib().swe_hip_rk_substep(self.handle, int(s)), "swe_hip_rk_substep")
+
+    def rk_steps(self, nsteps: int) -> None:
+        _chk(lib().swe_hip_rk_steps(self.handle, int(nsteps)), "swe_hip_rk_steps")
+
+    def eval_rhs(self, K: np.ndarray, member: int = 0) -> np.ndarray:
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        F = np.empty(4 * self.prob.compute_size)
+        _chk(lib().swe_hip_eval_rhs(self.handle, member, K.ctypes.data_as(_dp),
+                                    F.ctypes.data_as(_dp)), "swe_hip_eval_rhs")
+        return F
+
+    def set_timing(self, on: bool) -> None:
+        _chk(lib().swe_hip_set_timing(self.handle, int(bool(on))), "swe_hip_set_timing")
+
+    def timers(self, reset: bool = False) -> dict:
+        t = SweHipTimers()
+        _chk(lib().swe_hip_get_timers(self.handle, C.byref(t), int(bool(reset))),
+             "swe_hip_get_timers")
+        return {k: getattr(t, k) for k, _ in SweHipTimers._fields_}
+
+    def synchronize(self) -> None:
+        _chk(lib().swe_hip_synchronize(self.handle), "swe_hip_synchronize")
+
+    def state_device_ptr(self) -> int:
+        return lib().swe_hip_state_device_ptr(self.handle) or 0
+
+    def stream(self) -> int:
+        return lib().swe_hip_stream(self.handle) or 0
+
+    def device_bytes(self) -> int:
+        return int(lib().swe_hip_device_bytes(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            lib().swe_hip_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

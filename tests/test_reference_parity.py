@@ -1,0 +1,77 @@
+"""The C host and the CPU oracle against the reference build (oracle/_ref,
+the reference's own sources) on the reference's shipped 2562 input.
+Skipped where the reference tree is absent (the GPU box)."""
+import numpy as np
+import pytest
+
+from conftest import REFERENCE_BIN, have_reference
+
+pytestmark = pytest.mark.skipif(not have_reference(), reason="reference tree not present")
+
+LAYOUTS = ["default", "cfdl", "sfdl", "cfdl_sfdl"]
+ARRAYS = ["x", "y", "z", "f", "ghm", "p_u", "p_v", "p_w", "gradghm",
+          "Dx", "Dy", "Dz", "L", "idx", "H_init", "H_100"]
+
+
+@pytest.fixture(scope="module", params=[(lay, rcm) for lay in LAYOUTS for rcm in (True, False)],
+                ids=lambda p: f"{p[0]}-rcm{int(p[1])}")
+def pair(request, pkg, pyoracle):
+    lay, rcm = request.param
+    if not pyoracle.ref_available(lay):
+        pytest.skip("reference build missing")
+    ref = pyoracle.RefBuild(REFERENCE_BIN, use_rcm=rcm, layout=lay)
+    prob = pkg.host.Problem(REFERENCE_BIN, use_rcm=rcm, layout=lay, simd=4)
+    return ref, prob
+
+
+def test_host_prep_bitwise(pair):
+    ref, prob = pair
+    assert (prob.compute_size, prob.patch_size, prob.padded_nnbr) == \
+        (ref.compute_size, ref.patch_size, ref.padded_nnbr)
+    for nm in ARRAYS:
+        assert np.array_equal(getattr(prob, nm), getattr(ref, nm)), nm
+    if ref.mapping is None:
+        assert prob.mapping is None
+    else:
+        assert np.array_equal(prob.mapping, ref.mapping)
+
+
+def test_oracle_rhs_bitwise(pair, pyoracle):
+    ref, prob = pair
+    F_ref = ref.eval_rhs(ref.H_init)
+    F = pyoracle.Oracle(prob).eval_rhs(prob.H_init)
+    assert np.array_equal(F, F_ref)
+
+
+def test_oracle_100_steps_bitwise_and_known_answer(pair, pyoracle):
+    ref, prob = pair
+    H_ref = ref.run(ref.H_init, 100)
+    orc = pyoracle.Oracle(prob)
+    H = orc.run(prob.H_init, 100)
+    assert np.array_equal(H, H_ref)
+    # main.c:439-477 known-answer check at the FP64 tolerance (swe_config.h:88)
+    assert orc.verify(H, prob.H_100, 1e-9) == 0
+    assert prob.verify(H, prob.H_100, 1e-9) == 0
+
+
+def test_known_answer_error_profile(pkg, pyoracle):
+    """FP64 vs the file's H_100 (computed elsewhere, probably MATLAB): max
+    error in (1e-11, 1e-10]; without RCM two more values exceed 1e-11 -- the
+    two padded copies of the last node, which RCM moves away from the end."""
+    counts = {}
+    for rcm in (True, False):
+        prob = pkg.host.Problem(REFERENCE_BIN, use_rcm=rcm, layout="cfdl", simd=4)
+        orc = pyoracle.Oracle(prob)
+        H = orc.run(prob.H_init, 100)
+        assert orc.verify(H, prob.H_100, 1e-10) == 0
+        counts[rcm] = orc.verify(H, prob.H_100, 1e-11)
+    assert counts[True] > 0
+    assert counts[False] == counts[True] + 2
+
+
+def test_fast_oracle_close_to_strict(pkg, pyoracle):
+    prob = pkg.host.Problem(REFERENCE_BIN, use_rcm=True, layout="sfdl", simd=4)
+    a = pyoracle.Oracle(prob, "strict").run(prob.H_init, 100)
+    b = pyoracle.Oracle(prob, "fast").run(prob.H_init, 100)
+    from conftest import rel_by_var
+    assert (rel_by_var(b, a, prob) < 1e-12).all()
