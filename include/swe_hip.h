@@ -80,6 +80,21 @@ int swe_hip_create(const swe_hip_static *s, int members, unsigned flags,
                    int device, swe_hip_ctx **out);
 void swe_hip_destroy(swe_hip_ctx *ctx);
 
+/* Extended creation.  members/flags/device as swe_hip_create; member_block
+ * is the number of members one lane group of the ensemble kernel serves
+ * (a power of two 2..64 dividing members; 0 = the largest such value);
+ * share, when not NULL, is a context of the same mesh whose device copy of
+ * the static data is reused (s may then be NULL): the static data stay
+ * resident until the last context using them is destroyed. */
+typedef struct swe_hip_options {
+    int members;
+    unsigned flags;
+    int device;
+    int member_block;
+    swe_hip_ctx *share;
+} swe_hip_options;
+int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip_ctx **out);
+
 /* Per-attempt state upload (main.c:297 + 304-315): H in the host state
  * layout (SVM_LIN_ID, patch_size*4 values) into member slot `member`. */
 int swe_hip_set_state(swe_hip_ctx *ctx, int member, const double *H);
@@ -106,6 +121,17 @@ int swe_hip_set_timing(swe_hip_ctx *ctx, int on);
 int swe_hip_get_timers(swe_hip_ctx *ctx, swe_hip_timers *t, int reset);
 
 int swe_hip_synchronize(swe_hip_ctx *ctx);
+/* 1 when all work queued on the context's stream has finished, 0 while
+ * some is pending; never blocks. */
+int swe_hip_query(swe_hip_ctx *ctx);
+
+/* Ensemble slot management (members > 1): only member slots
+ * [0, active) are advanced by the following steps (lane groups wholly
+ * beyond `active` skip their work; the states of slots >= active are
+ * unspecified afterwards).  Stream-ordered, safe inside captured graphs.
+ * move_member copies the state H of slot `from` into slot `to`. */
+int swe_hip_set_active(swe_hip_ctx *ctx, int active);
+int swe_hip_move_member(swe_hip_ctx *ctx, int from, int to);
 
 /* Introspection for callers that drive the context's stream themselves
  * (benchmarks, the ensemble server): device state pointer of the layout
@@ -115,8 +141,10 @@ int swe_hip_info(const swe_hip_ctx *ctx, int *n_nodes, int *members,
                  int *nnbr, unsigned *flags);
 void *swe_hip_state_device_ptr(swe_hip_ctx *ctx);
 void *swe_hip_stream(swe_hip_ctx *ctx);
-/* Device bytes held by the context (static data + state buffers). */
+/* Device bytes held by the context's state buffers, and by the static
+ * data it uses (shared between contexts created with `share`). */
 long long swe_hip_device_bytes(const swe_hip_ctx *ctx);
+long long swe_hip_static_bytes(const swe_hip_ctx *ctx);
 
 const char *swe_hip_strerror(int code);
 const char *swe_hip_last_error(void);

@@ -1,0 +1,80 @@
+/*
+ * libswe_serve.so -- continuous-batching forecast server over libswe_hip.
+ *
+ * No counterpart in the reference (which runs one forecast per process,
+ * swe_code/main/main.c:294-392).  Requests (mesh, initial states of one or
+ * more ensemble members, step count) are queued as they arrive; meshes stay
+ * resident on the device; members are packed into batched contexts of the
+ * same mesh (one weight read serves every member of a batch); each batch
+ * runs on its own stream so batches overlap on the device; a member joins a
+ * batch at the next chunk boundary when a slot is free and the HBM budget
+ * allows, and leaves it as soon as its steps are done.  Every member's
+ * arithmetic is independent of the other members and of chunking, so a
+ * request's result is the same alone and in any mix.
+ *
+ * Threading: one host thread drives a server (submit / pump / completed).
+ * Host buffers passed to swe_srv_add_mesh and swe_srv_submit are borrowed:
+ * they must stay valid until the server is destroyed (mesh data) or the
+ * request completes (states).  Return codes are SWE_HIP_* (swe_hip.h).
+ */
+#ifndef SWE_SERVE_H
+#define SWE_SERVE_H
+
+#include "swe_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct swe_srv swe_srv;
+
+typedef struct swe_srv_config {
+    long long hbm_cap_bytes;   /* admission budget: static data + state buffers */
+    int slots_per_batch;       /* member slots of one batched context (even)    */
+    int member_block;          /* lane-group size of the ensemble kernel        */
+    int chunk_steps;           /* RK steps per launch chunk (admission period)  */
+    int max_batches_per_mesh;
+    unsigned flags;            /* SWE_HIP_EXACT                                 */
+    int device;
+} swe_srv_config;
+
+typedef struct swe_srv_stats {
+    long long member_steps_done;
+    long long requests_done, requests_pending;
+    long long hbm_in_use, hbm_peak;
+    int batches, meshes_resident;
+    long long chunks_launched;
+} swe_srv_stats;
+
+void swe_srv_default_config(swe_srv_config *cfg);
+int swe_srv_create(const swe_srv_config *cfg, swe_srv **out);
+void swe_srv_destroy(swe_srv *srv);
+
+/* Registers a mesh (static data in the host layout of swe_hip_static);
+ * its device copy is made at the first admission and then kept. */
+int swe_srv_add_mesh(swe_srv *srv, const swe_hip_static *s, int *mesh_id);
+
+/* Queues a request: init[m] / result[m] are host-layout states of member m
+ * (patch_size*4 doubles each).  Returns its id in *req_id. */
+int swe_srv_submit(swe_srv *srv, int mesh_id, int members, const double *const *init,
+                   double *const *result, int nsteps, long long tag, int *req_id);
+
+/* One non-blocking scheduling pass: retire members of batches whose chunk
+ * finished, admit queued members, launch the next chunks. *progress is set
+ * to the number of actions taken (0: nothing could be done right now). */
+int swe_srv_pump(swe_srv *srv, int *progress);
+
+/* Blocks until some in-flight chunk finishes (or returns at once if none). */
+int swe_srv_wait(swe_srv *srv);
+
+/* Drains up to max completion records: tag, latency (submit -> last member
+ * done) and queueing delay (submit -> first member admitted), seconds. */
+int swe_srv_completed(swe_srv *srv, long long *tags, double *latency_s, double *queue_s,
+                      int max, int *count);
+
+int swe_srv_get_stats(const swe_srv *srv, swe_srv_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -4,8 +4,9 @@ Modules:
   host     C host (libswe_host.so): reader, RCM, cfdl/sfdl layouts, serial patch
   runtime  HIP C-ABI (libswe_hip.so): device contexts, drop-in RK substeps,
            fused graph-captured steps, ensembles
+  serve    continuous-batching forecast server (libswe_serve.so)
   mesh     seeded generator of inputs in the reference's binary format
 """
-from . import host, mesh, runtime  # noqa: F401
+from . import host, mesh, runtime, serve  # noqa: F401
 
-__all__ = ["host", "mesh", "runtime"]
+__all__ = ["host", "mesh", "runtime", "serve"]

@@ -55,21 +55,38 @@ struct HostLayout {
 
 }  // namespace
 
-struct swe_hip_ctx {
+// Device copy of one patch's static data, shared by every context created
+// with `share` (reference-counted; owned by the contexts).
+struct SweMesh {
+    int refs = 1;
     int dev = 0;
-    hipStream_t stream = nullptr;
-    int n = 0, ld = 0, nnbr = 0, B = 1, MB = 1;
-    unsigned flags = 0;
-    HostLayout lay{1, 0, 1};
+    int n = 0, ld = 0, nnbr = 0;
     double gh0 = 0.0, dt = 0.0;
+    HostLayout lay{1, 0, 1};
     int *d_nbr = nullptr;
     double *d_wts = nullptr, *d_geo = nullptr;
+    long long bytes = 0;
+};
+
+struct swe_hip_ctx {
+    SweMesh *mesh = nullptr;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    // copies of the mesh fields used on every launch
+    int n = 0, ld = 0, nnbr = 0;
+    double gh0 = 0.0, dt = 0.0;
+    HostLayout lay{1, 0, 1};
+    int *d_nbr = nullptr;
+    double *d_wts = nullptr, *d_geo = nullptr;
+    int B = 1, MB = 1;
+    unsigned flags = 0;
     double *d_H = nullptr, *d_Ka = nullptr, *d_Kb = nullptr, *d_D = nullptr, *d_F = nullptr;
     double *d_stage = nullptr;   // [n*4] staging for member upload/read-back
+    int *d_active = nullptr;     // member slots advanced by the ensemble kernel
     hipGraphExec_t graph1 = nullptr, graphN = nullptr;
     bool timing = false;
     swe_hip_timers acc{};
-    long long bytes = 0;
+    long long bytes = 0;         // state buffers
     std::vector<double> host_tmp;
 };
 
@@ -148,6 +165,7 @@ void launch_fused(const swe_hip_ctx *c, int s)
 {
     DevStage st{};
     st.members = c->B;
+    st.active = c->d_active;
     st.h = c->d_H;
     st.d = c->d_D;
     const double dt = c->dt;
@@ -175,6 +193,7 @@ void launch_rhs(const swe_hip_ctx *c, const double *src, double *F)
 {
     DevStage st{};
     st.members = c->B;
+    st.active = c->d_active;
     st.src = src;
     st.h = src;
     st.out = F;
@@ -230,67 +249,58 @@ int capture_steps(swe_hip_ctx *c, int steps, hipGraphExec_t *exec)
     return 0;
 }
 
+void release_mesh(SweMesh *m)
+{
+    if (!m || --m->refs > 0)
+        return;
+    (void)hipSetDevice(m->dev);
+    if (m->d_nbr) (void)hipFree(m->d_nbr);
+    if (m->d_wts) (void)hipFree(m->d_wts);
+    if (m->d_geo) (void)hipFree(m->d_geo);
+    delete m;
+}
+
 void free_ctx(swe_hip_ctx *c)
 {
     if (!c)
         return;
     (void)hipSetDevice(c->dev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     if (c->graphN) (void)hipGraphExecDestroy(c->graphN);
-    void *ptrs[] = {c->d_nbr, c->d_wts, c->d_geo, c->d_H, c->d_Ka, c->d_Kb,
-                    c->d_D, c->d_F, c->d_stage};
+    void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    release_mesh(c->mesh);
     delete c;
 }
 
-int alloc(swe_hip_ctx *c, void **p, size_t bytes)
+int dev_alloc(long long *counter, void **p, size_t bytes)
 {
     hipError_t e = hipMalloc(p, bytes ? bytes : 8);
     if (e != hipSuccess)
         return fail(SWE_HIP_ENOMEM, std::string("hipMalloc(") + std::to_string(bytes)
                                         + "): " + hipGetErrorString(e));
-    c->bytes += (long long)bytes;
+    *counter += (long long)bytes;
     return 0;
 }
 
-int member_block(int B)
+int member_block(int B, int want)
 {
     if (B == 1)
         return 1;
+    if (want > 0)
+        return (want >= 2 && want <= 64 && (want & (want - 1)) == 0 && B % want == 0) ? want : -1;
     int mb = 64;
-    while (mb > 1 && (B % mb) != 0)
+    while (mb > 2 && (B % mb) != 0)
         mb >>= 1;
-    return mb;
+    return (B % mb) == 0 ? mb : -1;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char *swe_hip_strerror(int code)
+// Checks sizes, layout and pointers of the static data (no device use).
+int validate_static(const swe_hip_static *s)
 {
-    switch (code) {
-    case SWE_HIP_OK: return "success";
-    case SWE_HIP_EINVAL: return "invalid argument";
-    case SWE_HIP_ENOMEM: return "out of memory";
-    case SWE_HIP_EDEVICE: return "HIP device or runtime error";
-    case SWE_HIP_ERANGE: return "stencil index outside the patch";
-    default: return "unknown error";
-    }
-}
-
-const char *swe_hip_last_error(void) { return g_last_error.c_str(); }
-
-int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int device,
-                   swe_hip_ctx **out)
-{
-    if (!out)
-        return fail(SWE_HIP_EINVAL, "out is NULL");
-    *out = nullptr;
-    if (!s)
-        return fail(SWE_HIP_EINVAL, "static data is NULL");
     const int n = s->compute_size;
     if (n <= 0 || s->Nnbr < 1 || s->Nnbr > 64 || s->padded_Nnbr < s->Nnbr || s->simd < 1)
         return fail(SWE_HIP_EINVAL, "bad sizes (compute_size, Nnbr 1..64, padded_Nnbr, simd)");
@@ -300,17 +310,18 @@ int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int dev
                     "patch_size == compute_size): the whole sphere runs on one GPU");
     if ((!s->cfdl || s->sfdl) && n % s->simd != 0)
         return fail(SWE_HIP_EINVAL, "compute_size must be a multiple of SIMD_LENGTH");
-    if (members < 1 || members > 4096)
-        return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
     if (!s->x || !s->y || !s->z || !s->f || !s->ghm || !s->p_u || !s->p_v || !s->p_w
         || !s->gradghm || !s->idx || !s->Dx || !s->Dy || !s->Dz || !s->L)
         return fail(SWE_HIP_EINVAL, "a static array pointer is NULL");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-        return fail(SWE_HIP_EDEVICE, "no HIP device visible");
-    if (device < 0 || device >= ndev)
-        return fail(SWE_HIP_EINVAL, "device ordinal out of range");
+    return 0;
+}
 
+// Builds the device copy of validated static data; every stencil id is
+// range-checked on the host before anything reaches the device.
+int build_mesh(const swe_hip_static *s, int device, SweMesh **out)
+{
+    *out = nullptr;
+    const int n = s->compute_size;
     const HostLayout lay{s->cfdl ? 1 : 0, s->sfdl ? 1 : 0, s->simd};
     const int W = s->Nnbr, pW = s->padded_Nnbr;
     // Repack into the device layouts, checking every stencil id.
@@ -344,49 +355,125 @@ int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int dev
             geo[(size_t)(G_GG + c3) * n + i] = s->gradghm[v];
         }
     }
+    SweMesh *m = new SweMesh();
+    m->dev = device;
+    m->n = n;
+    m->ld = n;
+    m->nnbr = W;
+    m->gh0 = s->gh0;
+    m->dt = s->dt;
+    m->lay = lay;
+    int rc = 0;
+    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_nbr, sizeof(int) * nbr.size());
+    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_wts, sizeof(double) * wts.size());
+    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_geo, sizeof(double) * geo.size());
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpy(m->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess)
+        e = hipMemcpy(m->d_wts, wts.data(), sizeof(double) * wts.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess)
+        e = hipMemcpy(m->d_geo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice);
+    if (!rc && e != hipSuccess)
+        rc = fail(SWE_HIP_EDEVICE, std::string("static data upload: ") + hipGetErrorString(e));
+    if (rc) {
+        release_mesh(m);
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
 
+}  // namespace
+
+extern "C" {
+
+const char *swe_hip_strerror(int code)
+{
+    switch (code) {
+    case SWE_HIP_OK: return "success";
+    case SWE_HIP_EINVAL: return "invalid argument";
+    case SWE_HIP_ENOMEM: return "out of memory";
+    case SWE_HIP_EDEVICE: return "HIP device or runtime error";
+    case SWE_HIP_ERANGE: return "stencil index outside the patch";
+    default: return "unknown error";
+    }
+}
+
+const char *swe_hip_last_error(void) { return g_last_error.c_str(); }
+
+int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip_ctx **out)
+{
+    if (!out)
+        return fail(SWE_HIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!o)
+        return fail(SWE_HIP_EINVAL, "options are NULL");
+    if (!s && !o->share)
+        return fail(SWE_HIP_EINVAL, "static data is NULL");
+    if (o->members < 1 || o->members > 4096)
+        return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
+    if (!o->share) {
+        const int rc = validate_static(s);
+        if (rc)
+            return rc;
+    }
+    const int mb = member_block(o->members, o->member_block);
+    if (mb < 1)
+        return fail(SWE_HIP_EINVAL, "member_block must be a power of two in 2..64 dividing "
+                                    "members (members > 1 must be even)");
+    int device = o->device;
+    if (o->share)
+        device = o->share->dev;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(SWE_HIP_EDEVICE, "no HIP device visible");
+    if (device < 0 || device >= ndev)
+        return fail(SWE_HIP_EINVAL, "device ordinal out of range");
+    if (hipSetDevice(device) != hipSuccess)
+        return fail(SWE_HIP_EDEVICE, "hipSetDevice failed");
+    SweMesh *m = nullptr;
+    if (o->share) {
+        m = o->share->mesh;
+        m->refs++;
+    } else {
+        int rc = build_mesh(s, device, &m);
+        if (rc)
+            return rc;
+    }
     swe_hip_ctx *c = new swe_hip_ctx();
+    c->mesh = m;
     c->dev = device;
-    c->n = n;
-    c->ld = n;
-    c->nnbr = W;
-    c->B = members;
-    c->MB = member_block(members);
-    c->flags = flags;
-    c->lay = lay;
-    c->gh0 = s->gh0;
-    c->dt = s->dt;
+    c->n = m->n;
+    c->ld = m->ld;
+    c->nnbr = m->nnbr;
+    c->gh0 = m->gh0;
+    c->dt = m->dt;
+    c->lay = m->lay;
+    c->d_nbr = m->d_nbr;
+    c->d_wts = m->d_wts;
+    c->d_geo = m->d_geo;
+    c->B = o->members;
+    c->MB = mb;
+    c->flags = o->flags;
     int rc = 0;
     auto check = [&](hipError_t e, const char *what) {
         if (e != hipSuccess && !rc)
             rc = fail(SWE_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
     };
-    check(hipSetDevice(device), "hipSetDevice");
-    if (!rc)
-        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     const size_t sbytes = sizeof(double) * (size_t)state_count(c);
-    if (!rc) rc = alloc(c, (void **)&c->d_nbr, sizeof(int) * nbr.size());
-    if (!rc) rc = alloc(c, (void **)&c->d_wts, sizeof(double) * wts.size());
-    if (!rc) rc = alloc(c, (void **)&c->d_geo, sizeof(double) * geo.size());
-    if (!rc) rc = alloc(c, (void **)&c->d_H, sbytes);
-    if (!rc) rc = alloc(c, (void **)&c->d_Ka, sbytes);
-    if (!rc) rc = alloc(c, (void **)&c->d_Kb, sbytes);
-    if (!rc) rc = alloc(c, (void **)&c->d_D, sbytes);
-    if (!rc) rc = alloc(c, (void **)&c->d_F, sbytes);
-    if (!rc) rc = alloc(c, (void **)&c->d_stage, sizeof(double) * 4 * (size_t)n);
-    if (!rc)
-        check(hipMemcpy(c->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice),
-              "upload nbr");
-    if (!rc)
-        check(hipMemcpy(c->d_wts, wts.data(), sizeof(double) * wts.size(), hipMemcpyHostToDevice),
-              "upload wts");
-    if (!rc)
-        check(hipMemcpy(c->d_geo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice),
-              "upload geo");
-    double *bufs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F};
-    for (double *b : bufs)
+    double **bufs[] = {&c->d_H, &c->d_Ka, &c->d_Kb, &c->d_D, &c->d_F};
+    for (double **b : bufs)
+        if (!rc) rc = dev_alloc(&c->bytes, (void **)b, sbytes);
+    if (!rc) rc = dev_alloc(&c->bytes, (void **)&c->d_stage, sizeof(double) * 4 * (size_t)c->n);
+    if (!rc) rc = dev_alloc(&c->bytes, (void **)&c->d_active, sizeof(int));
+    for (double **b : bufs)
         if (!rc)
-            check(hipMemsetAsync(b, 0, sbytes, c->stream), "hipMemsetAsync");
+            check(hipMemsetAsync(*b, 0, sbytes, c->stream), "hipMemsetAsync");
+    if (!rc) {
+        k_set_int<<<1, 64, 0, c->stream>>>(c->d_active, c->B);
+        rc = check_launch("k_set_int");
+    }
     if (!rc)
         check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     if (rc) {
@@ -395,6 +482,13 @@ int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int dev
     }
     *out = c;
     return 0;
+}
+
+int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int device,
+                   swe_hip_ctx **out)
+{
+    swe_hip_options o{members, flags, device, 0, nullptr};
+    return swe_hip_create_ex(s, &o, out);
 }
 
 void swe_hip_destroy(swe_hip_ctx *ctx) { free_ctx(ctx); }
@@ -413,6 +507,40 @@ int swe_hip_info(const swe_hip_ctx *c, int *n_nodes, int *members, int *nnbr, un
 void *swe_hip_state_device_ptr(swe_hip_ctx *c) { return c ? (void *)c->d_H : nullptr; }
 void *swe_hip_stream(swe_hip_ctx *c) { return c ? (void *)c->stream : nullptr; }
 long long swe_hip_device_bytes(const swe_hip_ctx *c) { return c ? c->bytes : 0; }
+long long swe_hip_static_bytes(const swe_hip_ctx *c) { return c ? c->mesh->bytes : 0; }
+
+int swe_hip_query(swe_hip_ctx *c)
+{
+    if (!c)
+        return fail(SWE_HIP_EINVAL, "ctx is NULL");
+    hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess)
+        return 1;
+    if (e == hipErrorNotReady)
+        return 0;
+    return fail(SWE_HIP_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int swe_hip_set_active(swe_hip_ctx *c, int active)
+{
+    if (!c || active < 0 || active > c->B)
+        return fail(SWE_HIP_EINVAL, "set_active: bad ctx or count");
+    HIPCHK(hipSetDevice(c->dev));
+    k_set_int<<<1, 64, 0, c->stream>>>(c->d_active, active);
+    return check_launch("k_set_int");
+}
+
+int swe_hip_move_member(swe_hip_ctx *c, int from, int to)
+{
+    if (!c || from < 0 || to < 0 || from >= c->B || to >= c->B)
+        return fail(SWE_HIP_EINVAL, "move_member: bad ctx or slot");
+    if (from == to)
+        return 0;
+    HIPCHK(hipSetDevice(c->dev));
+    const long rows = 4L * c->n;
+    k_move_member<<<elementwise_blocks(rows), kBlock, 0, c->stream>>>(c->d_H, from, to, c->B, rows);
+    return check_launch("k_move_member");
+}
 
 static int to_device_member(swe_hip_ctx *c, int member, const double *Hhost, double *dst)
 {

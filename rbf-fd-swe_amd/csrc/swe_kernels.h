@@ -53,6 +53,7 @@ struct DevStage {
     double *out;        // next K, H (stage 3) or F (ST_RHS)
     double c;           // K = H + c*F (dt/2, dt/2, dt); stage 3: H += c*D (dt/6)
     double dcoef;       // D += dcoef*F (2, 2, 1)
+    const int *active;  // ensemble: member slots [0, *active) are advanced
     int members;        // B: state stride of a (node, var) row
     int nblk;           // grid.x, a multiple of 8
 };
@@ -251,6 +252,8 @@ template <int STAGE, bool EXACT, int MB, int NNBR>
 __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
 {
     static_assert(MB >= 2 && MB <= 64 && (MB & (MB - 1)) == 0, "MB power of two");
+    if ((int)blockIdx.y * MB >= *st.active)
+        return;   // whole member block idle (uniform over the workgroup)
     const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
     int node_raw = t / MB;
     if constexpr (MB == 64)
@@ -343,6 +346,21 @@ __global__ __launch_bounds__(256) void k_put_member(double *__restrict__ state,
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count)
         state[i * B + member] = flat[i];
+}
+
+__global__ void k_set_int(int *dst, int value)
+{
+    if (threadIdx.x == 0)
+        dst[threadIdx.x] = value;
+}
+
+// Copies member slot `from` to slot `to` for every (node, var) row.
+__global__ __launch_bounds__(256) void k_move_member(double *__restrict__ state, int from,
+                                                     int to, int B, long rows)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows)
+        state[i * B + to] = state[i * B + from];
 }
 
 __global__ __launch_bounds__(256) void k_get_member(double *__restrict__ flat,

@@ -22,7 +22,8 @@ EXPORTS = (
     "swe_hip_rk_substep", "swe_hip_rk_steps", "swe_hip_eval_rhs", "swe_hip_set_timing",
     "swe_hip_get_timers", "swe_hip_synchronize", "swe_hip_info",
     "swe_hip_state_device_ptr", "swe_hip_stream", "swe_hip_device_bytes",
-    "swe_hip_strerror", "swe_hip_last_error",
+    "swe_hip_strerror", "swe_hip_last_error", "swe_hip_create_ex", "swe_hip_query",
+    "swe_hip_set_active", "swe_hip_move_member", "swe_hip_static_bytes",
 )
 
 
@@ -34,6 +35,11 @@ class SweHipStatic(C.Structure):
                 ("x", _dp), ("y", _dp), ("z", _dp), ("f", _dp), ("ghm", _dp),
                 ("p_u", _dp), ("p_v", _dp), ("p_w", _dp), ("gradghm", _dp),
                 ("idx", _ip), ("Dx", _dp), ("Dy", _dp), ("Dz", _dp), ("L", _dp)]
+
+
+class SweHipOptions(C.Structure):
+    _fields_ = [("members", C.c_int), ("flags", C.c_uint), ("device", C.c_int),
+                ("member_block", C.c_int), ("share", C.c_void_p)]
 
 
 class SweHipTimers(C.Structure):
@@ -58,6 +64,13 @@ def lib() -> C.CDLL:
         vp = C.c_void_p
         L.swe_hip_create.argtypes = [C.POINTER(SweHipStatic), C.c_int, C.c_uint, C.c_int,
                                      C.POINTER(vp)]
+        L.swe_hip_create_ex.argtypes = [C.POINTER(SweHipStatic), C.POINTER(SweHipOptions),
+                                        C.POINTER(vp)]
+        L.swe_hip_query.argtypes = [vp]
+        L.swe_hip_set_active.argtypes = [vp, C.c_int]
+        L.swe_hip_move_member.argtypes = [vp, C.c_int, C.c_int]
+        L.swe_hip_static_bytes.argtypes = [vp]
+        L.swe_hip_static_bytes.restype = C.c_longlong
         L.swe_hip_destroy.argtypes = [vp]
         L.swe_hip_destroy.restype = None
         L.swe_hip_set_state.argtypes = [vp, C.c_int, _dp]
@@ -120,15 +133,21 @@ def static_from_problem(prob) -> tuple[SweHipStatic, list]:
 class HipContext:
     """One device context: static data of one mesh + `members` state slots."""
 
-    def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0):
+    def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0,
+                 member_block: int = 0, share: "HipContext | None" = None):
         self.prob = prob
         self.members = int(members)
         self.exact = bool(exact)
-        s, keep = static_from_problem(prob)
+        opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
+                             int(member_block), share.handle if share is not None else None)
         h = C.c_void_p()
-        _chk(lib().swe_hip_create(C.byref(s), self.members,
-                                  SWE_HIP_EXACT if exact else 0, device, C.byref(h)),
-             "swe_hip_create")
+        if share is not None:
+            rc = lib().swe_hip_create_ex(None, C.byref(opts), C.byref(h))
+        else:
+            s, keep = static_from_problem(prob)
+            rc = lib().swe_hip_create_ex(C.byref(s), C.byref(opts), C.byref(h))
+        _chk(rc, "swe_hip_create_ex")
+        self._share = share            # keeps the static data's owner alive
         self._h = h
         self.n_state = 4 * prob.patch_size
 
@@ -184,6 +203,21 @@ class HipContext:
 
     def device_bytes(self) -> int:
         return int(lib().swe_hip_device_bytes(self.handle))
+
+    def static_bytes(self) -> int:
+        return int(lib().swe_hip_static_bytes(self.handle))
+
+    def query(self) -> bool:
+        rc = lib().swe_hip_query(self.handle)
+        if rc < 0:
+            _chk(rc, "swe_hip_query")
+        return rc == 1
+
+    def set_active(self, active: int) -> None:
+        _chk(lib().swe_hip_set_active(self.handle, int(active)), "swe_hip_set_active")
+
+    def move_member(self, src: int, dst: int) -> None:
+        _chk(lib().swe_hip_move_member(self.handle, int(src), int(dst)), "swe_hip_move_member")
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:

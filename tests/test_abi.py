@@ -105,3 +105,35 @@ def test_driver_errors(pkg, tmp_path):
     env["SWE_USE_NETCDF"] = "1"
     r = subprocess.run([exe], env=env, capture_output=True, text=True)
     assert r.returncode == 1 and "NetCDF" in r.stdout
+
+
+def test_serve_library_exports_header(pkg):
+    so = pkg.serve.library_path()
+    names = _declared("swe_serve.h")
+    assert set(names) == set(pkg.serve.EXPORTS)
+    missing = [n for n in names if n not in _exported(so)]
+    assert not missing
+    L = pkg.serve.lib()
+    out = C.c_void_p()
+    assert L.swe_srv_create(None, C.byref(out)) == -1
+    cfg = pkg.serve.SrvConfig()
+    L.swe_srv_default_config(C.byref(cfg))
+    assert cfg.hbm_cap_bytes == 200 << 30 and cfg.slots_per_batch == 64
+    cfg.slots_per_batch = 3
+    assert L.swe_srv_create(C.byref(cfg), C.byref(out)) == -1
+
+
+def test_server_queues_without_device(pkg):
+    """Submitting and pumping before any device work: admission fails with a
+    device error only when a batch is actually needed."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    prob = pkg.host.Problem(pkg.mesh.case_path(642, seed=1), use_rcm=True, layout="cfdl")
+    srv = pkg.serve.Server(slots=4, member_block=2)
+    mid = srv.add_mesh(prob)
+    srv.submit(pkg.serve.Request(mesh=mid, inits=[prob.H_init], nsteps=3, tag=1))
+    assert srv.stats()["requests_pending"] == 1
+    with pytest.raises(pkg.runtime.SweHipError):
+        srv.pump()
+    srv.close()

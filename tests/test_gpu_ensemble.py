@@ -1,0 +1,64 @@
+"""Ensembles: every member of a batched launch equals its solo run."""
+import numpy as np
+import pytest
+
+from conftest import rel_by_var
+
+pytestmark = pytest.mark.gpu
+
+
+def _members(pkg, prob, B, seed=11):
+    case = pkg.mesh.read_bin(pkg.mesh.case_path(prob.n_nodes, seed=1))
+    E = pkg.mesh.perturb_members(case, B, seed=seed)
+    out = []
+    for m in range(B):
+        Hn = E[m][prob.mapping] if prob.mapping is not None else E[m]
+        full = np.concatenate([Hn, np.repeat(Hn[-1:], prob.padded - prob.n_nodes, 0)])
+        out.append(prob.from_node_major(full))
+    return out
+
+
+@pytest.mark.parametrize("B", [2, 4, 48, 64])
+def test_exact_members_equal_oracle(pkg, pyoracle, B):
+    prob = pkg.host.Problem(pkg.mesh.case_path(642, seed=1), use_rcm=True, layout="cfdl")
+    print(f"case: exact ensemble B={B} N=642", flush=True)
+    H0 = _members(pkg, prob, B)
+    orc = pyoracle.Oracle(prob)
+    with pkg.runtime.HipContext(prob, members=B, exact=True) as ctx:
+        for m in range(B):
+            ctx.set_state(H0[m], m)
+        ctx.rk_steps(5)
+        for m in sorted({0, 1, B // 2, B - 1}):
+            assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 5)), m
+
+
+def test_fast_batch_composition_invariance(pkg):
+    """A member's state after fused fast steps is bitwise the same whether it
+    runs alone (single-state kernel) or inside a 64-member batch."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
+    print("case: composition invariance B=64 N=2562", flush=True)
+    H0 = _members(pkg, prob, 64)
+    with pkg.runtime.HipContext(prob, members=64) as ctx:
+        for m in range(64):
+            ctx.set_state(H0[m], m)
+        ctx.rk_steps(20)
+        batch = [ctx.get_state(m) for m in (0, 37)]
+    for k, m in enumerate((0, 37)):
+        with pkg.runtime.HipContext(prob, members=1) as solo:
+            solo.set_state(H0[m])
+            solo.rk_steps(20)
+            assert np.array_equal(solo.get_state(), batch[k])
+
+
+def test_fast_members_vs_oracle(pkg, pyoracle):
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="sfdl")
+    print("case: fast ensemble B=16 N=2562 sfdl 50 steps", flush=True)
+    H0 = _members(pkg, prob, 16)
+    orc = pyoracle.Oracle(prob)
+    with pkg.runtime.HipContext(prob, members=16) as ctx:
+        for m in range(16):
+            ctx.set_state(H0[m], m)
+        ctx.rk_steps(50)
+        for m in (3, 15):
+            err = rel_by_var(ctx.get_state(m), orc.run(H0[m], 50), prob)
+            assert (err <= 1e-12).all(), (m, err)

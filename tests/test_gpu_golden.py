@@ -1,0 +1,57 @@
+"""HIP path against vectors made by the reference build itself
+(tests/golden, see make_golden.py): exact mode must match bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_by_var
+
+pytestmark = pytest.mark.gpu
+
+CASE = os.path.join(GOLDEN, "icos642_case.bin")
+REFV = os.path.join(GOLDEN, "icos642_ref.npz")
+
+
+def natural(prob, H):
+    Hn = prob.to_node_major(H)[: prob.n_nodes]
+    out = np.empty_like(Hn)
+    out[prob.mapping] = Hn
+    return out
+
+
+@pytest.fixture(scope="module")
+def g():
+    return np.load(REFV, allow_pickle=False)
+
+
+@pytest.mark.parametrize("layout", ["default", "cfdl", "sfdl", "cfdl_sfdl"])
+def test_rhs_and_steps_bitwise(pkg, g, layout):
+    prob = pkg.host.Problem(CASE, use_rcm=True, layout=layout, simd=4)
+    print(f"case: golden 642 layout={layout}", flush=True)
+    with pkg.runtime.HipContext(prob, exact=True) as ctx:
+        F = ctx.eval_rhs(prob.H_init)
+        assert np.array_equal(natural(prob, F), g["F0"])
+        ctx.set_state(prob.H_init)
+        for s in range(4):
+            ctx.rk_substep(s)
+        assert np.array_equal(natural(prob, ctx.get_state()), g["H1"])
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(10)
+        assert np.array_equal(natural(prob, ctx.get_state()), g["H10"])
+
+
+def test_known_answer_100_steps(pkg):
+    """main.c's verify_output on the reference-written H_100 slot."""
+    prob = pkg.host.Problem(CASE, use_rcm=True, layout="cfdl", simd=4)
+    with pkg.runtime.HipContext(prob, exact=True) as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(100)
+        H = ctx.get_state()
+    assert prob.verify(H, prob.H_100, 0.0) == 0
+    with pkg.runtime.HipContext(prob, exact=False) as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(100)
+        Hf = ctx.get_state()
+    assert prob.verify(Hf, prob.H_100, 1e-9) == 0
+    assert (rel_by_var(Hf, prob.H_100, prob) <= 1e-12).all()
