@@ -95,6 +95,14 @@ typedef struct swe_hip_options {
 } swe_hip_options;
 int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip_ctx **out);
 
+/* Autotuner of the ensemble kernel's member block: times `steps` fused RK
+ * steps of a `members`-slot context on the static data of `base` for every
+ * admissible member block (powers of two 2..64 dividing members) and
+ * returns the fastest in *best_mb with its time per step in
+ * *best_ms_per_step.  Timing uses HIP events on the candidates' streams. */
+int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, int steps,
+                              int *best_mb, double *best_ms_per_step);
+
 /* Per-attempt state upload (main.c:297 + 304-315): H in the host state
  * layout (SVM_LIN_ID, patch_size*4 values) into member slot `member`. */
 int swe_hip_set_state(swe_hip_ctx *ctx, int member, const double *H);

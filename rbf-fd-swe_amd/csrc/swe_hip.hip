@@ -493,6 +493,50 @@ int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int dev
 
 void swe_hip_destroy(swe_hip_ctx *ctx) { free_ctx(ctx); }
 
+int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, int steps,
+                              int *best_mb, double *best_ms)
+{
+    if (!base || members < 2 || steps < 1 || !best_mb)
+        return fail(SWE_HIP_EINVAL, "tune_member_block: bad arguments");
+    *best_mb = 0;
+    double best = 1e300;
+    for (int mb = 64; mb >= 2; mb >>= 1) {
+        if (members % mb)
+            continue;
+        swe_hip_options o{members, flags, base->dev, mb, base};
+        swe_hip_ctx *c = nullptr;
+        int rc = swe_hip_create_ex(nullptr, &o, &c);
+        if (rc)
+            return rc;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        float ms = 0.0f;
+        rc = swe_hip_rk_steps(c, 1);
+        hipError_t e = hipSuccess;
+        if (!rc) e = hipEventCreate(&e0);
+        if (!rc && e == hipSuccess) e = hipEventCreate(&e1);
+        if (!rc && e == hipSuccess) e = hipEventRecord(e0, c->stream);
+        if (!rc && e == hipSuccess) rc = swe_hip_rk_steps(c, steps);
+        if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        free_ctx(c);
+        if (!rc && e != hipSuccess)
+            rc = fail(SWE_HIP_EDEVICE, std::string("tune timing: ") + hipGetErrorString(e));
+        if (rc)
+            return rc;
+        const double per = (double)ms / steps;
+        if (per < best) {
+            best = per;
+            *best_mb = mb;
+        }
+    }
+    if (best_ms)
+        *best_ms = best;
+    return *best_mb ? 0 : fail(SWE_HIP_EINVAL, "members has no admissible member block");
+}
+
 int swe_hip_info(const swe_hip_ctx *c, int *n_nodes, int *members, int *nnbr, unsigned *flags)
 {
     if (!c)

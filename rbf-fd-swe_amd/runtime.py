@@ -24,6 +24,7 @@ EXPORTS = (
     "swe_hip_state_device_ptr", "swe_hip_stream", "swe_hip_device_bytes",
     "swe_hip_strerror", "swe_hip_last_error", "swe_hip_create_ex", "swe_hip_query",
     "swe_hip_set_active", "swe_hip_move_member", "swe_hip_static_bytes",
+    "swe_hip_tune_member_block",
 )
 
 
@@ -66,6 +67,8 @@ def lib() -> C.CDLL:
                                      C.POINTER(vp)]
         L.swe_hip_create_ex.argtypes = [C.POINTER(SweHipStatic), C.POINTER(SweHipOptions),
                                         C.POINTER(vp)]
+        L.swe_hip_tune_member_block.argtypes = [vp, C.c_int, C.c_uint, C.c_int,
+                                                C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.swe_hip_query.argtypes = [vp]
         L.swe_hip_set_active.argtypes = [vp, C.c_int]
         L.swe_hip_move_member.argtypes = [vp, C.c_int, C.c_int]
@@ -212,6 +215,17 @@ class HipContext:
         if rc < 0:
             _chk(rc, "swe_hip_query")
         return rc == 1
+
+    def tune_member_block(self, members: int, steps: int = 5, exact: bool = False):
+        """(best member block, ms per step) for a `members`-slot ensemble on
+        this context's mesh."""
+        mb = C.c_int()
+        ms = C.c_double()
+        _chk(lib().swe_hip_tune_member_block(self.handle, int(members),
+                                             SWE_HIP_EXACT if exact else 0, int(steps),
+                                             C.byref(mb), C.byref(ms)),
+             "swe_hip_tune_member_block")
+        return mb.value, ms.value
 
     def set_active(self, active: int) -> None:
         _chk(lib().swe_hip_set_active(self.handle, int(active)), "swe_hip_set_active")

@@ -62,3 +62,42 @@ def test_fast_members_vs_oracle(pkg, pyoracle):
         for m in (3, 15):
             err = rel_by_var(ctx.get_state(m), orc.run(H0[m], 50), prob)
             assert (err <= 1e-12).all(), (m, err)
+
+
+def test_autotuner_picks_admissible_block(pkg):
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
+    print("case: member-block autotuner B=64 N=2562", flush=True)
+    with pkg.runtime.HipContext(prob) as base:
+        mb, ms = base.tune_member_block(64, steps=3)
+        assert mb in (2, 4, 8, 16, 32, 64) and ms > 0.0
+        with pkg.runtime.HipContext(prob, members=64, member_block=mb, share=base) as ctx:
+            assert ctx.static_bytes() == base.static_bytes()
+            assert ctx.device_bytes() < base.static_bytes() * 4
+            ctx.set_state(prob.H_init, 5)
+            ctx.rk_steps(2)
+            base.set_state(prob.H_init)
+            base.rk_steps(2)
+            assert np.array_equal(ctx.get_state(5), base.get_state())
+
+
+def test_set_active_and_move_member(pkg):
+    """Slots beyond the active count are skipped; a moved member continues
+    exactly as if it had not moved."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(642, seed=1), use_rcm=True, layout="cfdl")
+    H0 = _members(pkg, prob, 4)
+    with pkg.runtime.HipContext(prob, members=8, member_block=2) as ctx:
+        for m in range(4):
+            ctx.set_state(H0[m], m)
+        ctx.set_state(H0[3], 7)
+        ctx.set_active(4)                 # member block of slots 6,7 idle
+        ctx.rk_steps(3)
+        frozen = ctx.get_state(7)
+        assert np.array_equal(frozen, H0[3])
+        ctx.move_member(3, 1)             # slot 1 <- slot 3
+        ctx.set_active(2)
+        ctx.rk_steps(2)
+        got = ctx.get_state(1)
+    with pkg.runtime.HipContext(prob) as solo:
+        solo.set_state(H0[3])
+        solo.rk_steps(5)
+        assert np.array_equal(solo.get_state(), got)
