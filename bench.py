@@ -55,6 +55,8 @@ def parse_args():
     ap.add_argument("--nodes", type=int, default=40962)
     ap.add_argument("--members", type=int, default=1)
     ap.add_argument("--member-block", type=int, default=0)
+    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane"])
+    ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--layout", default="cfdl")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -126,7 +128,8 @@ def main():
     print(f"bench: input ready N={prob.n_nodes} ({time.time() - t0:.1f}s)", file=sys.stderr,
           flush=True)
     ctx = pkg.runtime.HipContext(prob, members=args.members, exact=args.exact,
-                                 member_block=args.member_block)
+                                 member_block=args.member_block, kernel=args.kernel,
+                                 prefetch=args.prefetch)
     for m in range(args.members):
         ctx.set_state(prob.H_init, m)
     stream = torch.cuda.ExternalStream(ctx.stream())
@@ -177,7 +180,8 @@ def main():
         "config": {"workload": f"RBF-FD SWE RK4, N={prob.n_nodes} nodes, stencil 31, "
                                f"{args.members} member(s), fused stage kernels + HIP graphs",
                    "nodes": prob.n_nodes, "members": args.members, "layout": args.layout,
-                   "member_block": args.member_block,
+                   "member_block": args.member_block, "kernel": args.kernel,
+                   "prefetch": args.prefetch,
                    "exact": bool(args.exact), "dt": prob.dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -86,12 +86,18 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
  * share, when not NULL, is a context of the same mesh whose device copy of
  * the static data is reused (s may then be NULL): the static data stay
  * resident until the last context using them is destroyed. */
+#define SWE_HIP_KERNEL_AUTO 0
+#define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
+#define SWE_HIP_KERNEL_LANE 2   /* single state: 1 lane per node             */
 typedef struct swe_hip_options {
     int members;
     unsigned flags;
     int device;
     int member_block;
     swe_hip_ctx *share;
+    int kernel;                 /* SWE_HIP_KERNEL_* (members == 1 only)      */
+    int prefetch;               /* stencil slots per load batch: 0 (compiler
+                                   schedule), 4 or 8 (members == 1 only)   */
 } swe_hip_options;
 int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip_ctx **out);
 

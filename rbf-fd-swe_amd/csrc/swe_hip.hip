@@ -79,6 +79,8 @@ struct swe_hip_ctx {
     int *d_nbr = nullptr;
     double *d_wts = nullptr, *d_geo = nullptr;
     int B = 1, MB = 1;
+    int kernel = SWE_HIP_KERNEL_QUAD;   // single-state mapping
+    int prefetch = 0;                   // stencil slots per load batch (0, 4, 8)
     unsigned flags = 0;
     double *d_H = nullptr, *d_Ka = nullptr, *d_Kb = nullptr, *d_D = nullptr, *d_F = nullptr;
     double *d_stage = nullptr;   // [n*4] staging for member upload/read-back
@@ -120,15 +122,41 @@ template <int STAGE, bool EX, int NN>
 void launch_quad(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
     st.nblk = blocks_x(c, 4L * c->n);
-    k_stage_quad<STAGE, EX, NN><<<dim3(st.nblk), dim3(kBlock), 0, c->stream>>>(m, st);
+    const dim3 g(st.nblk), b(kBlock);
+    if (NN > 0 && c->prefetch == 8)
+        k_stage_quad<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
+    else if (NN > 0 && c->prefetch == 4)
+        k_stage_quad<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_quad<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
+}
+
+template <int STAGE, bool EX, int NN>
+void launch_lane(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    st.nblk = blocks_x(c, (long)c->n);
+    const dim3 g(st.nblk), b(kBlock);
+    if (NN > 0 && (EX || c->prefetch == 4))
+        k_stage_lane<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
+    else if (NN > 0 && c->prefetch == 8)
+        k_stage_lane<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_lane<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
 }
 
 template <int STAGE, bool EX, int MBv, int NN>
 void launch_members(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
     st.nblk = blocks_x(c, (long)c->n * MBv);
-    k_stage_members<STAGE, EX, MBv, NN>
-        <<<dim3(st.nblk, c->B / MBv), dim3(kBlock), 0, c->stream>>>(m, st);
+    const dim3 g(st.nblk, c->B / MBv), b(kBlock);
+    // Exact sums need the pipelined loop: left to the compiler, all the
+    // independent products are hoisted ahead of the serial sums.
+    if (NN > 0 && (EX || c->prefetch == 4))
+        k_stage_members<STAGE, EX, MBv, NN, 4><<<g, b, 0, c->stream>>>(m, st);
+    else if (NN > 0 && c->prefetch == 8)
+        k_stage_members<STAGE, EX, MBv, NN, 8><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_members<STAGE, EX, MBv, NN, 0><<<g, b, 0, c->stream>>>(m, st);
 }
 
 template <int STAGE, bool EX, int NN>
@@ -136,7 +164,12 @@ void launch_stage_nn(const swe_hip_ctx *c, const DevStage &st)
 {
     const DevMesh m = mesh_of(c);
     switch (c->MB) {
-    case 1: launch_quad<STAGE, EX, NN>(c, m, st); break;
+    case 1:
+        if (c->kernel == SWE_HIP_KERNEL_LANE)
+            launch_lane<STAGE, EX, NN>(c, m, st);
+        else
+            launch_quad<STAGE, EX, NN>(c, m, st);
+        break;
     case 2: launch_members<STAGE, EX, 2, NN>(c, m, st); break;
     case 4: launch_members<STAGE, EX, 4, NN>(c, m, st); break;
     case 8: launch_members<STAGE, EX, 8, NN>(c, m, st); break;
@@ -412,6 +445,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "static data is NULL");
     if (o->members < 1 || o->members > 4096)
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
+    if (o->prefetch != 0 && o->prefetch != 4 && o->prefetch != 8)
+        return fail(SWE_HIP_EINVAL, "prefetch must be 0, 4 or 8");
     if (!o->share) {
         const int rc = validate_static(s);
         if (rc)
@@ -454,6 +489,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->d_geo = m->d_geo;
     c->B = o->members;
     c->MB = mb;
+    c->kernel = o->kernel == SWE_HIP_KERNEL_LANE ? SWE_HIP_KERNEL_LANE : SWE_HIP_KERNEL_QUAD;
+    c->prefetch = o->prefetch;
     c->flags = o->flags;
     int rc = 0;
     auto check = [&](hipError_t e, const char *what) {
@@ -487,7 +524,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
 int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int device,
                    swe_hip_ctx **out)
 {
-    swe_hip_options o{members, flags, device, 0, nullptr};
+    swe_hip_options o{members, flags, device, 0, nullptr, 0, 0};
     return swe_hip_create_ex(s, &o, out);
 }
 
@@ -503,7 +540,7 @@ int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, in
     for (int mb = 64; mb >= 2; mb >>= 1) {
         if (members % mb)
             continue;
-        swe_hip_options o{members, flags, base->dev, mb, base};
+        swe_hip_options o{members, flags, base->dev, mb, base, 0, 0};
         swe_hip_ctx *c = nullptr;
         int rc = swe_hip_create_ex(nullptr, &o, &c);
         if (rc)

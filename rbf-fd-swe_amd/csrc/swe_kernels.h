@@ -181,13 +181,59 @@ __device__ __forceinline__ double quad_bcast(double v)
     return __hiloint2double(hi, lo);
 }
 
+// Stencil-slot loop shared by the stage kernels.  PF == 0 leaves the load
+// schedule to the compiler.  PF = D > 0 software-pipelines it: all neighbour
+// ids are loaded first, then a ring of D slots is kept in flight -- each
+// iteration consumes the oldest slot and issues the loads of slot j + D --
+// with scheduling barriers between iterations, so a wave always has D
+// gathers and 2*D weight loads outstanding and the compiler can neither
+// hoist further loads nor (in exact mode) precompute all 31x16 products
+// ahead of the serial sums, which would exhaust the registers.
+// Accumulation is sequential in j for every sum in all forms.
+template <int NNBR, int PF, typename LoadFn, typename AccFn, typename Slot>
+__device__ __forceinline__ void stencil_loop(const int *__restrict__ nb, long ld, int nn,
+                                             LoadFn load, AccFn acc, Slot *)
+{
+    if constexpr (NNBR > 0 && PF > 0) {
+        int q[NNBR];
+#pragma unroll
+        for (int j = 0; j < NNBR; ++j)
+            q[j] = nb[j * ld];
+        Slot ring[PF];
+#pragma unroll
+        for (int j = 0; j < PF && j < NNBR; ++j)
+            ring[j] = load(j, q[j]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < NNBR; ++j) {
+            acc(ring[j % PF]);
+            if (j + PF < NNBR)
+                ring[j % PF] = load(j + PF, q[j + PF]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else if constexpr (NNBR > 0) {
+#pragma unroll
+        for (int j = 0; j < NNBR; ++j)
+            acc(load(j, nb[j * ld]));
+    } else {
+#pragma unroll 4
+        for (int j = 0; j < nn; ++j)
+            acc(load(j, nb[j * ld]));
+    }
+}
+
+struct QuadSlot {
+    double2 w01, w23;
+    double s;
+};
+
 // ---------------------------------------------------------------------------
-// Single state (B == 1): four lanes per node.  Lane k of the quad reads
-// variable k of every neighbour and keeps the Dx, Dy, Dz, L sums of that
-// variable (each a sequential sum over the stencil); the 16 sums are then
-// shared across the quad by DPP, every lane evaluates the pointwise RHS and
-// lane k updates variable k.  256 threads = 64 nodes per workgroup.
-template <int STAGE, bool EXACT, int NNBR>
+// Single state (B == 1), quad mapping: four lanes per node.  Lane k of the
+// quad reads variable k of every neighbour and keeps the Dx, Dy, Dz, L sums
+// of that variable (each a sequential sum over the stencil); the 16 sums are
+// then shared across the quad by DPP, every lane evaluates the pointwise RHS
+// and lane k updates variable k.  256 threads = 64 nodes per workgroup.
+template <int STAGE, bool EXACT, int NNBR, int PF>
 __global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
 {
     const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
@@ -202,25 +248,20 @@ __global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
     const double *__restrict__ src = st.src;
 
     double sx = 0.0, sy = 0.0, sz = 0.0, sl = 0.0;
-    auto slot = [&](int j) {
-        const int q = nb[j * ld];
-        const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
-        const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
-        const double s = src[4L * q + k];
-        sx = madd<EXACT>(sx, w01.x, s);
-        sy = madd<EXACT>(sy, w01.y, s);
-        sz = madd<EXACT>(sz, w23.x, s);
-        sl = madd<EXACT>(sl, w23.y, s);
+    auto load = [&](int j, int q) {
+        QuadSlot r;
+        r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+        r.s = src[4L * q + k];
+        return r;
     };
-    if constexpr (NNBR > 0) {
-#pragma unroll
-        for (int j = 0; j < NNBR; ++j)
-            slot(j);
-    } else {
-#pragma unroll 4
-        for (int j = 0; j < nn; ++j)
-            slot(j);
-    }
+    auto acc = [&](const QuadSlot &r) {
+        sx = madd<EXACT>(sx, r.w01.x, r.s);
+        sy = madd<EXACT>(sy, r.w01.y, r.s);
+        sz = madd<EXACT>(sz, r.w23.x, r.s);
+        sl = madd<EXACT>(sl, r.w23.y, r.s);
+    };
+    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (QuadSlot *)nullptr);
     Sums S;
     S.dudx = quad_bcast<0>(sx); S.dudy = quad_bcast<0>(sy);
     S.dudz = quad_bcast<0>(sz); S.lu = quad_bcast<0>(sl);
@@ -241,6 +282,59 @@ __global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
         stage_update<STAGE, EXACT>(st, 4L * node + k, Fk);
 }
 
+struct LaneSlot {
+    double2 w01, w23, s01, s23;
+};
+
+// Single state (B == 1), lane mapping: one lane per node with all 16 sums.
+// A quarter of the quad kernel's load instructions per node (each lane
+// reads its own 32 B of weights and the neighbour's whole 32 B state), for
+// meshes large enough to fill the chip with one lane per node.
+template <int STAGE, bool EXACT, int NNBR, int PF>
+__global__ __launch_bounds__(256) void k_stage_lane(DevMesh m, DevStage st)
+{
+    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const bool live = t < m.n;
+    const int node = live ? t : m.n - 1;
+    const int nn = NNBR > 0 ? NNBR : m.nnbr;
+    const long ld = m.ld;
+    const int *__restrict__ nb = m.nbr + node;
+    const double *__restrict__ wp = m.wts + 4L * node;
+    const double *__restrict__ src = st.src;
+
+    Sums S = {};
+    auto load = [&](int j, int q) {
+        LaneSlot r;
+        r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+        r.s01 = *reinterpret_cast<const double2 *>(src + 4L * q);
+        r.s23 = *reinterpret_cast<const double2 *>(src + 4L * q + 2);
+        return r;
+    };
+    auto acc = [&](const LaneSlot &r) {
+        const double su = r.s01.x, sv = r.s01.y, sw = r.s23.x, sh = r.s23.y;
+        S.dudx = madd<EXACT>(S.dudx, r.w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, r.w01.x, sv);
+        S.dwdx = madd<EXACT>(S.dwdx, r.w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, r.w01.x, sh);
+        S.dudy = madd<EXACT>(S.dudy, r.w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, r.w01.y, sv);
+        S.dwdy = madd<EXACT>(S.dwdy, r.w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, r.w01.y, sh);
+        S.dudz = madd<EXACT>(S.dudz, r.w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, r.w23.x, sv);
+        S.dwdz = madd<EXACT>(S.dwdz, r.w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, r.w23.x, sh);
+        S.lu = madd<EXACT>(S.lu, r.w23.y, su); S.lv = madd<EXACT>(S.lv, r.w23.y, sv);
+        S.lw = madd<EXACT>(S.lw, r.w23.y, sw); S.lh = madd<EXACT>(S.lh, r.w23.y, sh);
+    };
+    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (LaneSlot *)nullptr);
+    const double2 s01 = *reinterpret_cast<const double2 *>(src + 4L * node);
+    const double2 s23 = *reinterpret_cast<const double2 *>(src + 4L * node + 2);
+    const Geo g = load_geo(m.geo, m.ld, node);
+    double F[4];
+    rhs_point(S, s01.x, s01.y, s23.x, s23.y, g, m.gh0, F);
+    if (live) {
+#pragma unroll
+        for (int var = 0; var < 4; ++var)
+            stage_update<STAGE, EXACT>(st, 4L * node + var, F[var]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Ensemble (B >= 2): lane = (node, member), MB members of one node per lane
 // group, member block blockIdx.y.  Each lane keeps all 16 sums of its
@@ -248,7 +342,12 @@ __global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
 // (wave-uniform scalar loads when MB == 64), so one weight read from HBM
 // serves every member of the block.  The neighbour state of a variable is a
 // contiguous run of MB doubles.
-template <int STAGE, bool EXACT, int MB, int NNBR>
+struct MemberSlot {
+    double2 w01, w23;
+    double su, sv, sw, sh;
+};
+
+template <int STAGE, bool EXACT, int MB, int NNBR, int PF>
 __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
 {
     static_assert(MB >= 2 && MB <= 64 && (MB & (MB - 1)) == 0, "MB power of two");
@@ -269,32 +368,31 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
     const double *__restrict__ src = st.src + mem;
 
     Sums S = {};
-    auto slot = [&](int j) {
-        int q = nb[j * ld];
+    auto load = [&](int j, int q) {
         if constexpr (MB == 64)
             q = __builtin_amdgcn_readfirstlane(q);
-        const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
-        const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+        MemberSlot r;
+        r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
         const double *sq = src + 4L * q * B;
-        const double su = sq[0], sv = sq[B], sw = sq[2 * B], sh = sq[3 * B];
-        S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
-        S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
-        S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
-        S.dwdy = madd<EXACT>(S.dwdy, w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, w01.y, sh);
-        S.dudz = madd<EXACT>(S.dudz, w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, w23.x, sv);
-        S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
-        S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
-        S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
+        r.su = sq[0];
+        r.sv = sq[B];
+        r.sw = sq[2 * B];
+        r.sh = sq[3 * B];
+        return r;
     };
-    if constexpr (NNBR > 0) {
-#pragma unroll
-        for (int j = 0; j < NNBR; ++j)
-            slot(j);
-    } else {
-#pragma unroll 4
-        for (int j = 0; j < nn; ++j)
-            slot(j);
-    }
+    auto acc = [&](const MemberSlot &r) {
+        const double su = r.su, sv = r.sv, sw = r.sw, sh = r.sh;
+        S.dudx = madd<EXACT>(S.dudx, r.w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, r.w01.x, sv);
+        S.dwdx = madd<EXACT>(S.dwdx, r.w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, r.w01.x, sh);
+        S.dudy = madd<EXACT>(S.dudy, r.w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, r.w01.y, sv);
+        S.dwdy = madd<EXACT>(S.dwdy, r.w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, r.w01.y, sh);
+        S.dudz = madd<EXACT>(S.dudz, r.w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, r.w23.x, sv);
+        S.dwdz = madd<EXACT>(S.dwdz, r.w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, r.w23.x, sh);
+        S.lu = madd<EXACT>(S.lu, r.w23.y, su); S.lv = madd<EXACT>(S.lv, r.w23.y, sv);
+        S.lw = madd<EXACT>(S.lw, r.w23.y, sw); S.lh = madd<EXACT>(S.lh, r.w23.y, sh);
+    };
+    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (MemberSlot *)nullptr);
     const double *own = src + 4L * node * B;
     const double u = own[0], v = own[B], w = own[2 * B], h = own[3 * B];
     const Geo g = load_geo(m.geo, m.ld, node);

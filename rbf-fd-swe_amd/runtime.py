@@ -15,6 +15,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 SWE_HIP_EXACT = 1
+KERNELS = {"auto": 0, "quad": 1, "lane": 2}
 
 # Every function include/swe_hip.h declares.
 EXPORTS = (
@@ -40,7 +41,8 @@ class SweHipStatic(C.Structure):
 
 class SweHipOptions(C.Structure):
     _fields_ = [("members", C.c_int), ("flags", C.c_uint), ("device", C.c_int),
-                ("member_block", C.c_int), ("share", C.c_void_p)]
+                ("member_block", C.c_int), ("share", C.c_void_p), ("kernel", C.c_int),
+                ("prefetch", C.c_int)]
 
 
 class SweHipTimers(C.Structure):
@@ -137,12 +139,14 @@ class HipContext:
     """One device context: static data of one mesh + `members` state slots."""
 
     def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0,
-                 member_block: int = 0, share: "HipContext | None" = None):
+                 member_block: int = 0, share: "HipContext | None" = None,
+                 kernel: str = "auto", prefetch: int = 0):
         self.prob = prob
         self.members = int(members)
         self.exact = bool(exact)
         opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
-                             int(member_block), share.handle if share is not None else None)
+                             int(member_block), share.handle if share is not None else None,
+                             KERNELS[kernel], int(prefetch))
         h = C.c_void_p()
         if share is not None:
             rc = lib().swe_hip_create_ex(None, C.byref(opts), C.byref(h))
