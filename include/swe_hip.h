@@ -89,6 +89,8 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
 #define SWE_HIP_KERNEL_AUTO 0
 #define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
 #define SWE_HIP_KERNEL_LANE 2   /* single state: 1 lane per node             */
+#define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
+#define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 typedef struct swe_hip_options {
     int members;
     unsigned flags;
@@ -96,8 +98,12 @@ typedef struct swe_hip_options {
     int member_block;
     swe_hip_ctx *share;
     int kernel;                 /* SWE_HIP_KERNEL_* (members == 1 only)      */
-    int prefetch;               /* stencil slots per load batch: 0 (compiler
-                                   schedule), 4 or 8 (members == 1 only)   */
+    int prefetch;               /* stencil slots in flight per wave: 0
+                                   (compiler schedule), 4 or 8              */
+    int node_order;             /* SWE_HIP_ORDER_* of the device copy (ignored
+                                   with share: the mesh's order is used)    */
+    int tiled;                  /* 1: LDS-staged ensemble kernel (members
+                                   >= 2, member_block 4, 8 or 16)           */
 } swe_hip_options;
 int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip_ctx **out);
 

@@ -1,0 +1,73 @@
+// Launchers of the single-state stage kernels (quad and lane mappings).
+#include "swe_ctx.h"
+
+namespace swe {
+namespace {
+
+template <int STAGE, bool EX, int NN>
+void launch_quad(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    st.nblk = blocks_x(c, 4L * c->n);
+    const dim3 g(st.nblk), b(kBlock);
+    if (NN > 0 && c->prefetch == 8)
+        k_stage_quad<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
+    else if (NN > 0 && c->prefetch == 4)
+        k_stage_quad<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_quad<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
+}
+
+template <int STAGE, bool EX, int NN>
+void launch_lane(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    st.nblk = blocks_x(c, (long)c->n);
+    const dim3 g(st.nblk), b(kBlock);
+    if (NN > 0 && (EX || c->prefetch == 4))
+        k_stage_lane<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
+    else if (NN > 0 && c->prefetch == 8)
+        k_stage_lane<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_lane<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
+}
+
+}  // namespace
+
+template <int STAGE, bool EX, int NN>
+void family_single(const swe_hip_ctx *c, const DevStage &st)
+{
+    const DevMesh m = mesh_of(c);
+    if (c->kernel == SWE_HIP_KERNEL_LANE)
+        launch_lane<STAGE, EX, NN>(c, m, st);
+    else
+        launch_quad<STAGE, EX, NN>(c, m, st);
+}
+
+namespace {
+
+template <int STAGE>
+void by_flags(const swe_hip_ctx *c, const DevStage &st)
+{
+    const bool nn31 = c->nnbr == 31;
+    if (exact(c)) {
+        if (nn31) family_single<STAGE, true, 31>(c, st);
+        else family_single<STAGE, true, 0>(c, st);
+    } else {
+        if (nn31) family_single<STAGE, false, 31>(c, st);
+        else family_single<STAGE, false, 0>(c, st);
+    }
+}
+
+}  // namespace
+
+void launch_single_stage(const swe_hip_ctx *c, int stage, const DevStage &st)
+{
+    switch (stage) {
+    case ST_0: by_flags<ST_0>(c, st); break;
+    case ST_1: by_flags<ST_1>(c, st); break;
+    case ST_2: by_flags<ST_2>(c, st); break;
+    case ST_3: by_flags<ST_3>(c, st); break;
+    default: by_flags<ST_RHS>(c, st); break;
+    }
+}
+
+}  // namespace swe

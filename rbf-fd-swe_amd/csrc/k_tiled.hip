@@ -1,0 +1,63 @@
+// Launchers of the LDS-staged (tiled) ensemble stage kernel.
+#include "swe_ctx.h"
+
+namespace swe {
+namespace {
+
+template <int STAGE, bool EX, int MBv, int NN>
+void launch_tiled(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    DevTiles tl;
+    tl.tile_start = c->tiles.d_tile_start;
+    tl.ustart = c->tiles.d_ustart;
+    tl.ulist = c->tiles.d_ulist;
+    tl.lidx = c->tiles.d_lidx;
+    tl.ntiles = c->tiles.ntiles;
+    tl.nmb = c->B / MBv;
+    st.nblk = (int)(((long)tl.ntiles * tl.nmb + 7) / 8 * 8);
+    const size_t lds = sizeof(double) * 4 * MBv * (size_t)c->tiles.umax;
+    k_stage_tiled<STAGE, EX, MBv, NN><<<dim3(st.nblk), dim3(kTiledBlock), lds, c->stream>>>(m, tl, st);
+}
+
+}  // namespace
+
+template <int STAGE, bool EX, int NN>
+void family_tiled(const swe_hip_ctx *c, const DevStage &st)
+{
+    const DevMesh m = mesh_of(c);
+    switch (c->MB) {
+    case 4: launch_tiled<STAGE, EX, 4, NN>(c, m, st); break;
+    case 8: launch_tiled<STAGE, EX, 8, NN>(c, m, st); break;
+    default: launch_tiled<STAGE, EX, 16, NN>(c, m, st); break;
+    }
+}
+
+namespace {
+
+template <int STAGE>
+void by_flags(const swe_hip_ctx *c, const DevStage &st)
+{
+    const bool nn31 = c->nnbr == 31;
+    if (exact(c)) {
+        if (nn31) family_tiled<STAGE, true, 31>(c, st);
+        else family_tiled<STAGE, true, 0>(c, st);
+    } else {
+        if (nn31) family_tiled<STAGE, false, 31>(c, st);
+        else family_tiled<STAGE, false, 0>(c, st);
+    }
+}
+
+}  // namespace
+
+void launch_tiled_stage(const swe_hip_ctx *c, int stage, const DevStage &st)
+{
+    switch (stage) {
+    case ST_0: by_flags<ST_0>(c, st); break;
+    case ST_1: by_flags<ST_1>(c, st); break;
+    case ST_2: by_flags<ST_2>(c, st); break;
+    case ST_3: by_flags<ST_3>(c, st); break;
+    default: by_flags<ST_RHS>(c, st); break;
+    }
+}
+
+}  // namespace swe

@@ -1,0 +1,128 @@
+// Internal definitions of libswe_hip shared by its translation units
+// (context and mesh structures, launch helpers).  Not part of the C ABI.
+#pragma once
+
+#include "swe_hip.h"
+#include "swe_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace swe {
+
+constexpr int kBlock = 256;            // threads per workgroup, stage kernels
+constexpr int kTiledBlock = 512;       // threads per workgroup, tiled kernel
+constexpr int kTiledLds = 80 * 1024;   // LDS per tiled workgroup: two per CU
+
+// Host index macros of the reference (swe_config.h:21-54) at run time.
+struct HostLayout {
+    int cfdl, sfdl, S;
+    long dm(int i, int j, int nnbr) const
+    {
+        return sfdl ? (long)(i / S) * S * nnbr + (long)j * S + i % S : (long)i * nnbr + j;
+    }
+    long vect(int i, int c) const
+    {
+        return sfdl ? (long)(i / S) * S * 3 + (long)c * S + i % S : (long)i * 3 + c;
+    }
+    long svm(int pid, int sv) const
+    {
+        return cfdl ? 4L * pid + sv : (long)(pid / S) * S * 4 + (long)sv * S + pid % S;
+    }
+};
+
+}  // namespace swe
+
+// Device copy of one patch's static data, shared by every context created
+// with `share` (reference-counted; owned by the contexts).
+struct SweMesh {
+    int refs = 1;
+    int dev = 0;
+    int n = 0, ld = 0, nnbr = 0;
+    double gh0 = 0.0, dt = 0.0;
+    swe::HostLayout lay{1, 0, 1};
+    int *d_nbr = nullptr;
+    double *d_wts = nullptr, *d_geo = nullptr;
+    long long bytes = 0;
+    int order = SWE_HIP_ORDER_HOST;
+    std::vector<int> perm;        // device node -> host node (empty: identity)
+    std::vector<int> h_nbr;       // host copy of nbr, device order, [nnbr][n]
+};
+
+// Tiles of the LDS-staged ensemble kernel (per context: they depend on the
+// member block through the LDS budget).
+struct SweTiles {
+    int ntiles = 0, umax = 0, tmax = 0;
+    int *d_tile_start = nullptr, *d_ustart = nullptr, *d_ulist = nullptr;
+    unsigned short *d_lidx = nullptr;
+};
+
+struct swe_hip_ctx {
+    SweMesh *mesh = nullptr;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    // copies of the mesh fields used on every launch
+    int n = 0, ld = 0, nnbr = 0;
+    double gh0 = 0.0, dt = 0.0;
+    swe::HostLayout lay{1, 0, 1};
+    int *d_nbr = nullptr;
+    double *d_wts = nullptr, *d_geo = nullptr;
+    int B = 1, MB = 1;
+    int kernel = SWE_HIP_KERNEL_QUAD;   // single-state mapping
+    int prefetch = 0;                   // stencil slots per load batch (0, 4, 8)
+    unsigned flags = 0;
+    double *d_H = nullptr, *d_Ka = nullptr, *d_Kb = nullptr, *d_D = nullptr, *d_F = nullptr;
+    double *d_stage = nullptr;   // [n*4] staging for member upload/read-back
+    int *d_active = nullptr;     // member slots advanced by the ensemble kernel
+    SweTiles tiles;              // LDS-staged ensemble kernel (ntiles == 0: unused)
+    hipGraphExec_t graph1 = nullptr, graphN = nullptr;
+    bool timing = false;
+    swe_hip_timers acc{};
+    long long bytes = 0;         // state buffers
+    std::vector<double> host_tmp;
+};
+
+namespace swe {
+
+inline bool exact(const swe_hip_ctx *c) { return (c->flags & SWE_HIP_EXACT) != 0; }
+
+inline long state_count(const swe_hip_ctx *c) { return 4L * c->n * c->B; }
+
+inline int blocks_x(const swe_hip_ctx *c, long threads)
+{
+    long nb = (threads + kBlock - 1) / kBlock;
+    nb = (nb + 7) / 8 * 8;   // multiple of 8 for the XCD remap
+    return (int)nb;
+}
+
+inline DevMesh mesh_of(const swe_hip_ctx *c)
+{
+    DevMesh m;
+    m.nbr = c->d_nbr;
+    m.wts = c->d_wts;
+    m.geo = c->d_geo;
+    m.n = c->n;
+    m.ld = c->ld;
+    m.nnbr = c->nnbr;
+    m.gh0 = c->gh0;
+    return m;
+}
+
+// One fused stage (ST_0..ST_3) or a plain RHS (ST_RHS) on the context's
+// stream, by kernel family (each in its own translation unit).
+void launch_single_stage(const swe_hip_ctx *c, int stage, const DevStage &st);
+void launch_members_stage(const swe_hip_ctx *c, int stage, const DevStage &st);
+void launch_tiled_stage(const swe_hip_ctx *c, int stage, const DevStage &st);
+
+inline void launch_stage(const swe_hip_ctx *c, int stage, const DevStage &st)
+{
+    if (c->tiles.ntiles > 0)
+        launch_tiled_stage(c, stage, st);
+    else if (c->B == 1)
+        launch_single_stage(c, stage, st);
+    else
+        launch_members_stage(c, stage, st);
+}
+
+}  // namespace swe

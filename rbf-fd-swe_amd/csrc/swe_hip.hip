@@ -3,18 +3,87 @@
 // driver (RK_substep semantics, swe_code/main/rk4_rbffd_swe.c:30-186), the
 // fused and graph-captured fast path, state upload/read-back in the
 // reference's host layouts (swe_code/main/include/swe_config.h:21-54).
-#include "swe_hip.h"
-#include "swe_kernels.h"
+#include "swe_ctx.h"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
 using namespace swe;
+
+namespace swe {
+
+// ---------------------------------------------------------------------------
+// Element-wise kernels of the drop-in stage driver (rk4_rbffd_swe.c:74-186),
+// over `count` contiguous values.
+__global__ __launch_bounds__(256) void k_copy(double *__restrict__ dst,
+                                              const double *__restrict__ src, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        dst[i] = src[i];
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_axpy_inplace(double *__restrict__ y, double a,
+                                                      const double *__restrict__ x, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        y[i] = axpy<EXACT>(y[i], a, x[i]);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_axpy_to(double *__restrict__ out,
+                                                 const double *__restrict__ y, double a,
+                                                 const double *__restrict__ x, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        out[i] = axpy<EXACT>(y[i], a, x[i]);
+}
+
+// Member slot <-> node-major [node][var] staging copy.
+__global__ __launch_bounds__(256) void k_put_member(double *__restrict__ state,
+                                                    const double *__restrict__ flat,
+                                                    int member, int B, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        state[i * B + member] = flat[i];
+}
+
+__global__ void k_set_int(int *dst, int value)
+{
+    if (threadIdx.x == 0)
+        dst[threadIdx.x] = value;
+}
+
+// Copies member slot `from` to slot `to` for every (node, var) row.
+__global__ __launch_bounds__(256) void k_move_member(double *__restrict__ state, int from,
+                                                     int to, int B, long rows)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows)
+        state[i * B + to] = state[i * B + from];
+}
+
+__global__ __launch_bounds__(256) void k_get_member(double *__restrict__ flat,
+                                                    const double *__restrict__ state,
+                                                    int member, int B, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        flat[i] = state[i * B + member];
+}
+
+}  // namespace swe
 
 namespace {
 
@@ -33,164 +102,11 @@ int fail(int code, const std::string &msg)
             return fail(SWE_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kBlock = 256;
 constexpr int kGraphSteps = 8;   // RK steps per captured graph
-
-// Host index macros of the reference (swe_config.h:21-54) at run time.
-struct HostLayout {
-    int cfdl, sfdl, S;
-    long dm(int i, int j, int nnbr) const
-    {
-        return sfdl ? (long)(i / S) * S * nnbr + (long)j * S + i % S : (long)i * nnbr + j;
-    }
-    long vect(int i, int c) const
-    {
-        return sfdl ? (long)(i / S) * S * 3 + (long)c * S + i % S : (long)i * 3 + c;
-    }
-    long svm(int pid, int sv) const
-    {
-        return cfdl ? 4L * pid + sv : (long)(pid / S) * S * 4 + (long)sv * S + pid % S;
-    }
-};
 
 }  // namespace
 
-// Device copy of one patch's static data, shared by every context created
-// with `share` (reference-counted; owned by the contexts).
-struct SweMesh {
-    int refs = 1;
-    int dev = 0;
-    int n = 0, ld = 0, nnbr = 0;
-    double gh0 = 0.0, dt = 0.0;
-    HostLayout lay{1, 0, 1};
-    int *d_nbr = nullptr;
-    double *d_wts = nullptr, *d_geo = nullptr;
-    long long bytes = 0;
-};
-
-struct swe_hip_ctx {
-    SweMesh *mesh = nullptr;
-    int dev = 0;
-    hipStream_t stream = nullptr;
-    // copies of the mesh fields used on every launch
-    int n = 0, ld = 0, nnbr = 0;
-    double gh0 = 0.0, dt = 0.0;
-    HostLayout lay{1, 0, 1};
-    int *d_nbr = nullptr;
-    double *d_wts = nullptr, *d_geo = nullptr;
-    int B = 1, MB = 1;
-    int kernel = SWE_HIP_KERNEL_QUAD;   // single-state mapping
-    int prefetch = 0;                   // stencil slots per load batch (0, 4, 8)
-    unsigned flags = 0;
-    double *d_H = nullptr, *d_Ka = nullptr, *d_Kb = nullptr, *d_D = nullptr, *d_F = nullptr;
-    double *d_stage = nullptr;   // [n*4] staging for member upload/read-back
-    int *d_active = nullptr;     // member slots advanced by the ensemble kernel
-    hipGraphExec_t graph1 = nullptr, graphN = nullptr;
-    bool timing = false;
-    swe_hip_timers acc{};
-    long long bytes = 0;         // state buffers
-    std::vector<double> host_tmp;
-};
-
 namespace {
-
-bool exact(const swe_hip_ctx *c) { return (c->flags & SWE_HIP_EXACT) != 0; }
-
-long state_count(const swe_hip_ctx *c) { return 4L * c->n * c->B; }
-
-int blocks_x(const swe_hip_ctx *c, long threads)
-{
-    long nb = (threads + kBlock - 1) / kBlock;
-    nb = (nb + 7) / 8 * 8;   // multiple of 8 for the XCD remap
-    return (int)nb;
-}
-
-DevMesh mesh_of(const swe_hip_ctx *c)
-{
-    DevMesh m;
-    m.nbr = c->d_nbr;
-    m.wts = c->d_wts;
-    m.geo = c->d_geo;
-    m.n = c->n;
-    m.ld = c->ld;
-    m.nnbr = c->nnbr;
-    m.gh0 = c->gh0;
-    return m;
-}
-
-template <int STAGE, bool EX, int NN>
-void launch_quad(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
-{
-    st.nblk = blocks_x(c, 4L * c->n);
-    const dim3 g(st.nblk), b(kBlock);
-    if (NN > 0 && c->prefetch == 8)
-        k_stage_quad<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
-    else if (NN > 0 && c->prefetch == 4)
-        k_stage_quad<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
-    else
-        k_stage_quad<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
-}
-
-template <int STAGE, bool EX, int NN>
-void launch_lane(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
-{
-    st.nblk = blocks_x(c, (long)c->n);
-    const dim3 g(st.nblk), b(kBlock);
-    if (NN > 0 && (EX || c->prefetch == 4))
-        k_stage_lane<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
-    else if (NN > 0 && c->prefetch == 8)
-        k_stage_lane<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
-    else
-        k_stage_lane<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
-}
-
-template <int STAGE, bool EX, int MBv, int NN>
-void launch_members(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
-{
-    st.nblk = blocks_x(c, (long)c->n * MBv);
-    const dim3 g(st.nblk, c->B / MBv), b(kBlock);
-    // Exact sums need the pipelined loop: left to the compiler, all the
-    // independent products are hoisted ahead of the serial sums.
-    if (NN > 0 && (EX || c->prefetch == 4))
-        k_stage_members<STAGE, EX, MBv, NN, 4><<<g, b, 0, c->stream>>>(m, st);
-    else if (NN > 0 && c->prefetch == 8)
-        k_stage_members<STAGE, EX, MBv, NN, 8><<<g, b, 0, c->stream>>>(m, st);
-    else
-        k_stage_members<STAGE, EX, MBv, NN, 0><<<g, b, 0, c->stream>>>(m, st);
-}
-
-template <int STAGE, bool EX, int NN>
-void launch_stage_nn(const swe_hip_ctx *c, const DevStage &st)
-{
-    const DevMesh m = mesh_of(c);
-    switch (c->MB) {
-    case 1:
-        if (c->kernel == SWE_HIP_KERNEL_LANE)
-            launch_lane<STAGE, EX, NN>(c, m, st);
-        else
-            launch_quad<STAGE, EX, NN>(c, m, st);
-        break;
-    case 2: launch_members<STAGE, EX, 2, NN>(c, m, st); break;
-    case 4: launch_members<STAGE, EX, 4, NN>(c, m, st); break;
-    case 8: launch_members<STAGE, EX, 8, NN>(c, m, st); break;
-    case 16: launch_members<STAGE, EX, 16, NN>(c, m, st); break;
-    case 32: launch_members<STAGE, EX, 32, NN>(c, m, st); break;
-    default: launch_members<STAGE, EX, 64, NN>(c, m, st); break;
-    }
-}
-
-template <int STAGE>
-void launch_stage(const swe_hip_ctx *c, const DevStage &st)
-{
-    const bool nn31 = c->nnbr == 31;
-    if (exact(c)) {
-        if (nn31) launch_stage_nn<STAGE, true, 31>(c, st);
-        else launch_stage_nn<STAGE, true, 0>(c, st);
-    } else {
-        if (nn31) launch_stage_nn<STAGE, false, 31>(c, st);
-        else launch_stage_nn<STAGE, false, 0>(c, st);
-    }
-}
 
 // Fused stage s of one RK step: gathers from H (s=0), Ka (s=1), Kb (s=2),
 // Ka (s=3); writes Ka, Kb, Ka, H.
@@ -205,19 +121,19 @@ void launch_fused(const swe_hip_ctx *c, int s)
     switch (s) {
     case 0:
         st.src = c->d_H; st.out = c->d_Ka; st.c = dt / 2.0; st.dcoef = 0.0;
-        launch_stage<ST_0>(c, st);
+        launch_stage(c, ST_0, st);
         break;
     case 1:
         st.src = c->d_Ka; st.out = c->d_Kb; st.c = dt / 2.0; st.dcoef = 2.0;
-        launch_stage<ST_1>(c, st);
+        launch_stage(c, ST_1, st);
         break;
     case 2:
         st.src = c->d_Kb; st.out = c->d_Ka; st.c = dt; st.dcoef = 2.0;
-        launch_stage<ST_2>(c, st);
+        launch_stage(c, ST_2, st);
         break;
     default:
         st.src = c->d_Ka; st.out = c->d_H; st.c = dt / 6.0; st.dcoef = 1.0;
-        launch_stage<ST_3>(c, st);
+        launch_stage(c, ST_3, st);
         break;
     }
 }
@@ -230,7 +146,7 @@ void launch_rhs(const swe_hip_ctx *c, const double *src, double *F)
     st.src = src;
     st.h = src;
     st.out = F;
-    launch_stage<ST_RHS>(c, st);
+    launch_stage(c, ST_RHS, st);
 }
 
 int elementwise_blocks(long count) { return (int)((count + kBlock - 1) / kBlock); }
@@ -301,7 +217,9 @@ void free_ctx(swe_hip_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     if (c->graphN) (void)hipGraphExecDestroy(c->graphN);
-    void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active};
+    void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active,
+                    c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
+                    c->tiles.d_lidx};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -351,44 +269,84 @@ int validate_static(const swe_hip_static *s)
 
 // Builds the device copy of validated static data; every stencil id is
 // range-checked on the host before anything reaches the device.
-int build_mesh(const swe_hip_static *s, int device, SweMesh **out)
+// 3-D Morton code of a point of the unit cube [-1, 1]^3 (21 bits per axis).
+uint64_t morton3(double x, double y, double z)
+{
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    auto q = [](double a) {
+        double t = (a + 1.0) * 0.5 * 2097151.0;
+        t = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
+        return (uint64_t)t;
+    };
+    return spread(q(x)) | spread(q(y)) << 1 | spread(q(z)) << 2;
+}
+
+int build_mesh(const swe_hip_static *s, int device, int order, SweMesh **out)
 {
     *out = nullptr;
     const int n = s->compute_size;
     const HostLayout lay{s->cfdl ? 1 : 0, s->sfdl ? 1 : 0, s->simd};
     const int W = s->Nnbr, pW = s->padded_Nnbr;
+    // Device node order: the host's (RCM) order, or a Morton curve through
+    // the node coordinates (compact tiles for the LDS-staged kernel).  The
+    // arithmetic of every node is the same in either order.
+    std::vector<int> perm, inv;
+    if (order == SWE_HIP_ORDER_MORTON) {
+        std::vector<std::pair<uint64_t, int>> key(n);
+        for (int i = 0; i < n; ++i)
+            key[i] = {morton3(s->x[i], s->y[i], s->z[i]), i};
+        std::sort(key.begin(), key.end());
+        perm.resize(n);
+        inv.resize(n);
+        for (int d = 0; d < n; ++d) {
+            perm[d] = key[d].second;
+            inv[key[d].second] = d;
+        }
+    }
+    auto host_of = [&](int d) { return perm.empty() ? d : perm[d]; };
+    auto dev_of = [&](int h) { return inv.empty() ? h : inv[h]; };
     // Repack into the device layouts, checking every stencil id.
     std::vector<int> nbr((size_t)W * n);
     std::vector<double> wts((size_t)W * n * 4);
     std::vector<double> geo((size_t)GEO_N * n);
-    for (int i = 0; i < n; ++i) {
+    for (int dnode = 0; dnode < n; ++dnode) {
+        const int i = host_of(dnode);
         for (int j = 0; j < W; ++j) {
             const long d = lay.dm(i, j, pW);
             const int q = s->idx[d];
             if (q < 0 || q >= n)
                 return fail(SWE_HIP_ERANGE, "idx[" + std::to_string(d) + "] = " + std::to_string(q)
                                                 + " outside [0, " + std::to_string(n) + ")");
-            nbr[(size_t)j * n + i] = q;
-            double *w = &wts[((size_t)j * n + i) * 4];
+            nbr[(size_t)j * n + dnode] = dev_of(q);
+            double *w = &wts[((size_t)j * n + dnode) * 4];
             w[0] = s->Dx[d];
             w[1] = s->Dy[d];
             w[2] = s->Dz[d];
             w[3] = s->L[d];
         }
-        geo[(size_t)G_X * n + i] = s->x[i];
-        geo[(size_t)G_Y * n + i] = s->y[i];
-        geo[(size_t)G_Z * n + i] = s->z[i];
-        geo[(size_t)G_F * n + i] = s->f[i];
-        geo[(size_t)G_GHM * n + i] = s->ghm[i];
+        geo[(size_t)G_X * n + dnode] = s->x[i];
+        geo[(size_t)G_Y * n + dnode] = s->y[i];
+        geo[(size_t)G_Z * n + dnode] = s->z[i];
+        geo[(size_t)G_F * n + dnode] = s->f[i];
+        geo[(size_t)G_GHM * n + dnode] = s->ghm[i];
         for (int c3 = 0; c3 < 3; ++c3) {
             const long v = lay.vect(i, c3);
-            geo[(size_t)(G_PU + c3) * n + i] = s->p_u[v];
-            geo[(size_t)(G_PV + c3) * n + i] = s->p_v[v];
-            geo[(size_t)(G_PW + c3) * n + i] = s->p_w[v];
-            geo[(size_t)(G_GG + c3) * n + i] = s->gradghm[v];
+            geo[(size_t)(G_PU + c3) * n + dnode] = s->p_u[v];
+            geo[(size_t)(G_PV + c3) * n + dnode] = s->p_v[v];
+            geo[(size_t)(G_PW + c3) * n + dnode] = s->p_w[v];
+            geo[(size_t)(G_GG + c3) * n + dnode] = s->gradghm[v];
         }
     }
     SweMesh *m = new SweMesh();
+    m->order = order;
     m->dev = device;
     m->n = n;
     m->ld = n;
@@ -412,8 +370,75 @@ int build_mesh(const swe_hip_static *s, int device, SweMesh **out)
         release_mesh(m);
         return rc;
     }
+    m->perm = std::move(perm);
+    m->h_nbr = std::move(nbr);
     *out = m;
     return 0;
+}
+
+// Greedy tiles along the device node order: a tile grows while it has fewer
+// than tmax nodes and its distinct stencil nodes fit umax LDS rows.
+int build_tiles(swe_hip_ctx *c, int tmax, int umax)
+{
+    const int n = c->n, W = c->nnbr;
+    const std::vector<int> &nbr = c->mesh->h_nbr;
+    std::vector<int> tstart{0}, ustart{0}, ulist;
+    std::vector<unsigned short> lidx((size_t)W * n);
+    std::vector<int> mark(n, -1), pos(n, 0);
+    std::vector<int> cur;
+    int tile = 0, node = 0;
+    while (node < n) {
+        cur.clear();
+        const int first = node;
+        while (node < n && node - first < tmax) {
+            int fresh = 0;
+            for (int j = 0; j < W; ++j) {
+                const int q = nbr[(size_t)j * n + node];
+                if (mark[q] != tile) {
+                    mark[q] = tile;
+                    cur.push_back(q);
+                    ++fresh;
+                }
+            }
+            if ((int)cur.size() > umax) {
+                if (node == first)
+                    return fail(SWE_HIP_EINVAL, "one stencil exceeds the LDS tile budget");
+                for (int k = 0; k < fresh; ++k)
+                    mark[cur.back()] = -1, cur.pop_back();
+                break;
+            }
+            ++node;
+        }
+        std::sort(cur.begin(), cur.end());
+        for (int k = 0; k < (int)cur.size(); ++k)
+            pos[cur[k]] = k;
+        for (int i = first; i < node; ++i)
+            for (int j = 0; j < W; ++j)
+                lidx[(size_t)j * n + i] = (unsigned short)pos[nbr[(size_t)j * n + i]];
+        ulist.insert(ulist.end(), cur.begin(), cur.end());
+        tstart.push_back(node);
+        ustart.push_back((int)ulist.size());
+        ++tile;
+    }
+    SweTiles &t = c->tiles;
+    t.ntiles = tile;
+    t.tmax = tmax;
+    t.umax = umax;
+    int rc = 0;
+    if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_tile_start, sizeof(int) * tstart.size());
+    if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_ustart, sizeof(int) * ustart.size());
+    if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_ulist, sizeof(int) * ulist.size());
+    if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_lidx, sizeof(unsigned short) * lidx.size());
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpy(t.d_tile_start, tstart.data(), sizeof(int) * tstart.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemcpy(t.d_ustart, ustart.data(), sizeof(int) * ustart.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemcpy(t.d_ulist, ulist.data(), sizeof(int) * ulist.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemcpy(t.d_lidx, lidx.data(), sizeof(unsigned short) * lidx.size(), hipMemcpyHostToDevice);
+    if (!rc && e != hipSuccess)
+        rc = fail(SWE_HIP_EDEVICE, std::string("tile upload: ") + hipGetErrorString(e));
+    if (rc)
+        t.ntiles = 0;
+    return rc;
 }
 
 }  // namespace
@@ -447,6 +472,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
     if (o->prefetch != 0 && o->prefetch != 4 && o->prefetch != 8)
         return fail(SWE_HIP_EINVAL, "prefetch must be 0, 4 or 8");
+    if (o->node_order != SWE_HIP_ORDER_HOST && o->node_order != SWE_HIP_ORDER_MORTON)
+        return fail(SWE_HIP_EINVAL, "node_order must be SWE_HIP_ORDER_HOST or _MORTON");
     if (!o->share) {
         const int rc = validate_static(s);
         if (rc)
@@ -471,7 +498,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         m = o->share->mesh;
         m->refs++;
     } else {
-        int rc = build_mesh(s, device, &m);
+        int rc = build_mesh(s, device, o->node_order, &m);
         if (rc)
             return rc;
     }
@@ -511,6 +538,13 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         k_set_int<<<1, 64, 0, c->stream>>>(c->d_active, c->B);
         rc = check_launch("k_set_int");
     }
+    if (!rc && o->tiled) {
+        if (c->B < 2 || (c->MB != 4 && c->MB != 8 && c->MB != 16))
+            rc = fail(SWE_HIP_EINVAL, "the tiled ensemble kernel needs members >= 2 and a "
+                                      "member block of 4, 8 or 16");
+        else
+            rc = build_tiles(c, kTiledBlock / c->MB, kTiledLds / (32 * c->MB));
+    }
     if (!rc)
         check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     if (rc) {
@@ -524,7 +558,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
 int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int device,
                    swe_hip_ctx **out)
 {
-    swe_hip_options o{members, flags, device, 0, nullptr, 0, 0};
+    swe_hip_options o{members, flags, device, 0, nullptr, 0, 0, 0, 0};
     return swe_hip_create_ex(s, &o, out);
 }
 
@@ -540,7 +574,7 @@ int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, in
     for (int mb = 64; mb >= 2; mb >>= 1) {
         if (members % mb)
             continue;
-        swe_hip_options o{members, flags, base->dev, mb, base, 0, 0};
+        swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, 0};
         swe_hip_ctx *c = nullptr;
         int rc = swe_hip_create_ex(nullptr, &o, &c);
         if (rc)
@@ -628,9 +662,12 @@ static int to_device_member(swe_hip_ctx *c, int member, const double *Hhost, dou
     const int n = c->n;
     c->host_tmp.resize(4 * (size_t)n);
     double *t = c->host_tmp.data();
-    for (int i = 0; i < n; ++i)
+    const std::vector<int> &perm = c->mesh->perm;
+    for (int d = 0; d < n; ++d) {
+        const int i = perm.empty() ? d : perm[d];
         for (int v = 0; v < 4; ++v)
-            t[4 * (size_t)i + v] = Hhost[c->lay.svm(i, v)];
+            t[4 * (size_t)d + v] = Hhost[c->lay.svm(i, v)];
+    }
     HIPCHK(hipSetDevice(c->dev));
     if (c->B == 1) {
         HIPCHK(hipMemcpyAsync(dst, t, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->stream));
@@ -665,9 +702,12 @@ static int from_device_member(swe_hip_ctx *c, int member, const double *src, dou
                               c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (int i = 0; i < n; ++i)
+    const std::vector<int> &perm = c->mesh->perm;
+    for (int d = 0; d < n; ++d) {
+        const int i = perm.empty() ? d : perm[d];
         for (int v = 0; v < 4; ++v)
-            Hhost[c->lay.svm(i, v)] = t[4 * (size_t)i + v];
+            Hhost[c->lay.svm(i, v)] = t[4 * (size_t)d + v];
+    }
     return 0;
 }
 

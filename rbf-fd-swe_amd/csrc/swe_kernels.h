@@ -407,67 +407,89 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
 }
 
 // ---------------------------------------------------------------------------
-// Element-wise kernels of the drop-in stage driver (rk4_rbffd_swe.c:74-186),
-// over `count` contiguous values.
-__global__ __launch_bounds__(256) void k_copy(double *__restrict__ dst,
-                                              const double *__restrict__ src, long count)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count)
-        dst[i] = src[i];
-}
+// Ensemble, tiled (B >= 2): the mesh is stored in a space-filling-curve node
+// order and cut into tiles of consecutive nodes whose stencils share few
+// neighbours (a 64-node tile touches ~220 distinct nodes, 9x reuse).  One
+// workgroup = one tile x MB members: it first copies the tile's distinct
+// neighbour rows (4 variables x MB members each) from HBM/L2 into LDS, then
+// lane (node, member) gathers its 31 neighbours from LDS.  Member blocks of
+// one tile are consecutive workgroups on the same XCD, so the tile's weights
+// come from HBM once and from L2 for the other member blocks.
+struct DevTiles {
+    const int *tile_start;          // [ntiles + 1] first node of each tile
+    const int *ustart;              // [ntiles + 1] offsets into ulist
+    const int *ulist;               // distinct stencil nodes of each tile, ascending
+    const unsigned short *lidx;     // [nnbr][ld] position of nbr(j, node) in its tile's list
+    int ntiles;
+    int nmb;                        // member blocks per tile (B / MB)
+};
 
-template <bool EXACT>
-__global__ __launch_bounds__(256) void k_axpy_inplace(double *__restrict__ y, double a,
-                                                      const double *__restrict__ x, long count)
+template <int STAGE, bool EXACT, int MB, int NNBR>
+__global__ __launch_bounds__(512) void k_stage_tiled(DevMesh m, DevTiles tl, DevStage st)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count)
-        y[i] = axpy<EXACT>(y[i], a, x[i]);
-}
-
-template <bool EXACT>
-__global__ __launch_bounds__(256) void k_axpy_to(double *__restrict__ out,
-                                                 const double *__restrict__ y, double a,
-                                                 const double *__restrict__ x, long count)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count)
-        out[i] = axpy<EXACT>(y[i], a, x[i]);
-}
-
-// Member slot <-> node-major [node][var] staging copy.
-__global__ __launch_bounds__(256) void k_put_member(double *__restrict__ state,
-                                                    const double *__restrict__ flat,
-                                                    int member, int B, long count)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count)
-        state[i * B + member] = flat[i];
-}
-
-__global__ void k_set_int(int *dst, int value)
-{
-    if (threadIdx.x == 0)
-        dst[threadIdx.x] = value;
-}
-
-// Copies member slot `from` to slot `to` for every (node, var) row.
-__global__ __launch_bounds__(256) void k_move_member(double *__restrict__ state, int from,
-                                                     int to, int B, long rows)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < rows)
-        state[i * B + to] = state[i * B + from];
-}
-
-__global__ __launch_bounds__(256) void k_get_member(double *__restrict__ flat,
-                                                    const double *__restrict__ state,
-                                                    int member, int B, long count)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count)
-        flat[i] = state[i * B + member];
+    static_assert(MB >= 2 && MB <= 64 && (MB & (MB - 1)) == 0, "MB power of two");
+    extern __shared__ double rows[];   // [ucount][4][MB]
+    const int lb = xcd_block(st.nblk);
+    const int tile = lb / tl.nmb;
+    const int mb = lb - tile * tl.nmb;
+    if (tile >= tl.ntiles || mb * MB >= *st.active)
+        return;   // uniform over the workgroup
+    const int t0 = tl.tile_start[tile];
+    const int tcount = tl.tile_start[tile + 1] - t0;
+    const int u0 = tl.ustart[tile];
+    const int ucount = tl.ustart[tile + 1] - u0;
+    const long B = st.members;
+    const int col0 = mb * MB;
+    const double *__restrict__ src = st.src;
+    const int total = ucount * 4 * MB;
+    for (int i = (int)threadIdx.x; i < total; i += (int)blockDim.x) {
+        const int r = i / MB;
+        const int c = i & (MB - 1);
+        const int u = tl.ulist[u0 + (r >> 2)];
+        rows[i] = src[(4L * u + (r & 3)) * B + col0 + c];
+    }
+    __syncthreads();
+    const int nl = (int)threadIdx.x / MB;
+    const int ml = (int)threadIdx.x & (MB - 1);
+    if (nl >= tcount)
+        return;   // no barrier follows
+    const int node = t0 + nl;
+    const int nn = NNBR > 0 ? NNBR : m.nnbr;
+    const long ld = m.ld;
+    const unsigned short *__restrict__ lx = tl.lidx + node;
+    const double *__restrict__ wp = m.wts + 4L * node;
+    Sums S = {};
+    auto slot = [&](int j) {
+        const double *sp = rows + (int)lx[j * ld] * 4 * MB + ml;
+        const double su = sp[0], sv = sp[MB], sw = sp[2 * MB], sh = sp[3 * MB];
+        const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+        S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
+        S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
+        S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
+        S.dwdy = madd<EXACT>(S.dwdy, w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, w01.y, sh);
+        S.dudz = madd<EXACT>(S.dudz, w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, w23.x, sv);
+        S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
+        S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
+        S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
+    };
+    if constexpr (NNBR > 0) {
+#pragma unroll 8
+        for (int j = 0; j < NNBR; ++j)
+            slot(j);
+    } else {
+#pragma unroll 4
+        for (int j = 0; j < nn; ++j)
+            slot(j);
+    }
+    const long e = 4L * node * B + col0 + ml;
+    const double u = src[e], v = src[e + B], w = src[e + 2 * B], h = src[e + 3 * B];
+    const Geo g = load_geo(m.geo, m.ld, node);
+    double F[4];
+    rhs_point(S, u, v, w, h, g, m.gh0, F);
+#pragma unroll
+    for (int var = 0; var < 4; ++var)
+        stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
 }
 
 }  // namespace swe

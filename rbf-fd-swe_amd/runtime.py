@@ -16,6 +16,7 @@ _ip = C.POINTER(C.c_int)
 
 SWE_HIP_EXACT = 1
 KERNELS = {"auto": 0, "quad": 1, "lane": 2}
+ORDERS = {"host": 0, "morton": 1}
 
 # Every function include/swe_hip.h declares.
 EXPORTS = (
@@ -42,7 +43,7 @@ class SweHipStatic(C.Structure):
 class SweHipOptions(C.Structure):
     _fields_ = [("members", C.c_int), ("flags", C.c_uint), ("device", C.c_int),
                 ("member_block", C.c_int), ("share", C.c_void_p), ("kernel", C.c_int),
-                ("prefetch", C.c_int)]
+                ("prefetch", C.c_int), ("node_order", C.c_int), ("tiled", C.c_int)]
 
 
 class SweHipTimers(C.Structure):
@@ -140,13 +141,14 @@ class HipContext:
 
     def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0,
                  member_block: int = 0, share: "HipContext | None" = None,
-                 kernel: str = "auto", prefetch: int = 0):
+                 kernel: str = "auto", prefetch: int = 0, order: str = "host",
+                 tiled: bool = False):
         self.prob = prob
         self.members = int(members)
         self.exact = bool(exact)
         opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
                              int(member_block), share.handle if share is not None else None,
-                             KERNELS[kernel], int(prefetch))
+                             KERNELS[kernel], int(prefetch), ORDERS[order], int(bool(tiled)))
         h = C.c_void_p()
         if share is not None:
             rc = lib().swe_hip_create_ex(None, C.byref(opts), C.byref(h))

@@ -101,3 +101,55 @@ def test_set_active_and_move_member(pkg):
         solo.set_state(H0[3])
         solo.rk_steps(5)
         assert np.array_equal(solo.get_state(), got)
+
+
+@pytest.mark.parametrize("B,mb", [(8, 4), (16, 8), (32, 16), (64, 8)])
+def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb):
+    """LDS-staged kernel over a Morton-ordered device copy: bitwise equal to
+    the oracle (node order and staging never change a node's arithmetic)."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="sfdl")
+    print(f"case: tiled exact B={B} mb={mb} N=2562", flush=True)
+    H0 = _members(pkg, prob, B)
+    orc = pyoracle.Oracle(prob)
+    with pkg.runtime.HipContext(prob, members=B, member_block=mb, exact=True,
+                                order="morton", tiled=True) as ctx:
+        for m in range(B):
+            ctx.set_state(H0[m], m)
+        ctx.rk_steps(4)
+        for m in sorted({0, B // 3, B - 1}):
+            assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 4)), m
+
+
+def test_tiled_fast_composition_and_active(pkg):
+    prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl")
+    print("case: tiled fast B=64 N=10242 vs solo", flush=True)
+    H0 = _members(pkg, prob, 64)
+    with pkg.runtime.HipContext(prob, members=64, member_block=16, order="morton",
+                                tiled=True) as ctx:
+        for m in range(64):
+            ctx.set_state(H0[m], m)
+        ctx.rk_steps(12)
+        got = {m: ctx.get_state(m) for m in (0, 17, 63)}
+        ctx.set_active(16)
+        ctx.rk_steps(3)
+        got15 = ctx.get_state(15)
+    with pkg.runtime.HipContext(prob) as solo:
+        for m in (0, 17, 63):
+            solo.set_state(H0[m])
+            solo.rk_steps(12)
+            assert np.array_equal(solo.get_state(), got[m]), m
+        solo.set_state(H0[15])
+        solo.rk_steps(15)
+        assert np.array_equal(solo.get_state(), got15)
+
+
+@pytest.mark.parametrize("kernel,pf", [("quad", 0), ("quad", 8), ("lane", 0), ("lane", 4)])
+def test_single_state_variants_exact(pkg, pyoracle, kernel, pf):
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
+    print(f"case: single-state {kernel} prefetch={pf} morton exact", flush=True)
+    want = pyoracle.Oracle(prob).run(prob.H_init, 3)
+    with pkg.runtime.HipContext(prob, exact=True, kernel=kernel, prefetch=pf,
+                                order="morton") as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(3)
+        assert np.array_equal(ctx.get_state(), want)
