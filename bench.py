@@ -60,6 +60,12 @@ def parse_args():
     ap.add_argument("--layout", default="cfdl")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the ensemble leg (N=163842, 64 members, 500 steps)")
+    ap.add_argument("--serve", action="store_true",
+                    help="add the serving leg (Poisson mix of forecast requests)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=25.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -105,6 +111,99 @@ def run_reference_cpu(path: str, n_nodes: int, target_s: float):
                        f"s/step averaged over attempts >= 2 = {per_step:.4e} s "
                        f"({wall:.1f} s wall)"),
             "verified": "VERIFICATION" in out}
+
+
+def ensemble_leg(pkg, torch, steps=500):
+    """64-member ensemble at N=163842 (tiled kernel, autotuned member block)
+    against the single-state path on the same mesh."""
+    import numpy as np
+    prob = pkg.host.Problem(pkg.mesh.case_path(163842, seed=1), use_rcm=True, layout="cfdl")
+    prob.path = pkg.mesh.case_path(163842, seed=1)
+    inits = pkg.serve.member_pool(pkg.mesh, prob, 64, seed=4)
+
+    def timed(ctx, k):
+        st = torch.cuda.ExternalStream(ctx.stream())
+        ctx.rk_steps(2)
+        ctx.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        ctx.rk_steps(k)
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / k
+    single = pkg.runtime.HipContext(prob)
+    single.set_state(inits[0])
+    ms1 = timed(single, steps)
+    base = pkg.runtime.HipContext(prob, order="morton")
+    best = None
+    for mb in (4, 8, 16):
+        ctx = pkg.runtime.HipContext(prob, members=64, member_block=mb, share=base, tiled=True)
+        for m in range(64):
+            ctx.set_state(inits[m], m)
+        ms = timed(ctx, 20)
+        if best is None or ms < best[1]:
+            best = (mb, ms)
+        ctx.close()
+    ens = pkg.runtime.HipContext(prob, members=64, member_block=best[0], share=base, tiled=True)
+    for m in range(64):
+        ens.set_state(inits[m], m)
+    ms = timed(ens, steps)
+    finite = bool(np.isfinite(ens.get_state(63)).all())
+    ens.close()
+    base.close()
+    single.close()
+    n = prob.n_nodes
+    return {"workload": f"N={n}, 64 members (seeded 1e-3 perturbations), {steps} RK4 steps, "
+                        "LDS-tiled ensemble kernel on a Morton-ordered device copy",
+            "member_steps_per_s": 64 / (ms * 1e-3), "ms_per_step": ms,
+            "member_block": best[0], "gnode_rhs_per_s": n * 4 * 64 / (ms * 1e-3) / 1e9,
+            "fp64_tflops": n * 64 * 4 * 1058 / (ms * 1e-3) / 1e12,
+            "single_member_steps_per_s": 1 / (ms1 * 1e-3),
+            "ratio_vs_single": (64 / ms) / (1 / ms1), "state_finite": finite}
+
+
+def serve_leg(pkg, n_requests, rate, seed=2026):
+    """Continuous-batching mix: Poisson arrivals, N in {2562, 10242, 40962,
+    163842}, 1-32 members, 50-500 steps, HBM cap 200 GB."""
+    import numpy as np
+    probs, pools = [], []
+    for n in (2562, 10242, 40962, 163842):
+        path = pkg.mesh.case_path(n, seed=1)
+        p = pkg.host.Problem(path, use_rcm=True, layout="cfdl")
+        p.path = path
+        probs.append(p)
+        pools.append(pkg.serve.member_pool(pkg.mesh, p, 32, seed=n))
+    reqs = pkg.serve.poisson_mix(pools, n_requests, rate, seed)
+    lat, que, done = [], [], [0]
+
+    def on_done(r):
+        lat.append(r.latency_s)
+        que.append(r.queue_s)
+        if not all(np.isfinite(x[:8]).all() for x in r.results):
+            raise RuntimeError(f"request {r.tag} returned non-finite state")
+        r.results = None
+        r.inits = None
+        done[0] += 1
+        if done[0] % 200 == 0:
+            print(f"bench: serve {done[0]}/{len(reqs)} done", file=sys.stderr, flush=True)
+    work = sum(len(r.inits) * r.nsteps for r in reqs)
+    node_rhs = sum(len(r.inits) * r.nsteps * 4 * probs[r.mesh].n_nodes for r in reqs)
+    with pkg.serve.Server(hbm_cap_bytes=200 << 30, slots=64, member_block=16,
+                          chunk_steps=16) as srv:
+        for p in probs:
+            srv.add_mesh(p)
+        stats = srv.run(reqs, on_done=on_done)
+    lat = np.array(lat)
+    return {"workload": f"{n_requests} requests, Poisson {rate}/s, N in {{2562,10242,40962,163842}}, "
+                        "1-32 members, 50-500 steps, HBM cap 200 GB",
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "p50_queue_s": float(np.percentile(np.array(que), 50)),
+            "member_steps_per_s": work / stats["makespan_s"],
+            "gnode_rhs_per_s": node_rhs / stats["makespan_s"] / 1e9,
+            "makespan_s": stats["makespan_s"], "hbm_peak_gb": stats["hbm_peak"] / 2**30,
+            "batches": stats["batches"], "chunks": stats["chunks_launched"]}
 
 
 def main():
@@ -161,9 +260,16 @@ def main():
             traffic = json.load(fp).get("hbm_bytes_per_launch")
     node_rhs = float(prob.n_nodes) * 4 * args.steps * args.members
     value = node_rhs / (gpu_ms * 1e-3) / 1e9
+    ctx.close()
     cpu = None
     if not args.no_cpu:
         cpu = run_reference_cpu(path, prob.n_nodes, args.cpu_seconds)
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = ensemble_leg(pkg, torch)
+    serving = None
+    if args.serve:
+        serving = serve_leg(pkg, args.serve_requests, args.serve_rate)
     out = {
         "metric": "eval_RHS GNodes/s + HBM GB/s; ensemble steps/s and p50/p99 latency",
         "value": value,
@@ -189,11 +295,12 @@ def main():
                      "bytes_per_launch": stage_bytes(N, args.members),
                      "avg_launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,
+        "ensemble": ensemble,
+        "serving": serving,
         "member_steps_per_s": args.members * args.steps / (gpu_ms * 1e-3),
         "wall_s_timed_region": wall,
         "state_finite": finite,
     }
-    ctx.close()
     print(json.dumps(out), flush=True)
 
 

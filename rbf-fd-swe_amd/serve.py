@@ -198,3 +198,37 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def member_pool(pkg_mesh, prob, count: int, seed: int) -> list:
+    """`count` seeded ensemble initial states of one mesh, host layout of
+    `prob` (perturbed TC5-like states from the generator)."""
+    case = pkg_mesh.read_bin(prob.path)
+    E = pkg_mesh.perturb_members(case, count, seed=seed)
+    out = []
+    for Hn in E:
+        if prob.mapping is not None:
+            Hn = Hn[prob.mapping]
+        full = np.concatenate([Hn, np.repeat(Hn[-1:], prob.padded - prob.n_nodes, 0)])
+        out.append(prob.from_node_major(full))
+    return out
+
+
+def poisson_mix(pools: list, n_requests: int, rate_per_s: float, seed: int,
+                members=(1, 32), steps=(50, 500)) -> list:
+    """Requests with Poisson arrivals: mesh uniform over the pools, members
+    and step count uniform in the given inclusive ranges; member states
+    drawn from the mesh's pool."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for tag in range(n_requests):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        mid = int(rng.integers(len(pools)))
+        B = int(rng.integers(members[0], members[1] + 1))
+        pool = pools[mid]
+        first = int(rng.integers(len(pool)))
+        inits = [pool[(first + m) % len(pool)] for m in range(B)]
+        reqs.append(Request(mesh=mid, inits=inits, nsteps=int(rng.integers(steps[0], steps[1] + 1)),
+                            tag=tag, arrival_s=t))
+    return reqs
