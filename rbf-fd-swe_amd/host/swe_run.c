@@ -12,6 +12,12 @@
  *                    main.c:336-358) | steps (fused, graph-captured)
  *   SWE_HIP_EXACT    1 = bitwise reference arithmetic (default 0)
  *   SWE_MEMBERS      ensemble members advanced together (default 1)
+ *   SWE_OUTPUT_FILE, SWE_HIST_INT   history output, the reference's USE_HIST
+ *                    parameters (runtime_params.c:24-27, a stub there): the
+ *                    state of member 0 every SWE_HIST_INT steps of the first
+ *                    attempt, in the file node order (RCM and padding undone):
+ *                    int32 N, then per snapshot int32 step, f64 time,
+ *                    f64 H[N][4] (u, v, w, h)
  * Init pipeline main.c:94-190, attempt loop :294-392, verify_output
  * :439-477 (every member against H_100), profiling report profiling.c.
  * Errors: a missing parameter or a NetCDF request prints and exits 1, an
@@ -21,6 +27,7 @@
 #include "swe_hip.h"
 #include "swe_host.h"
 
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -41,6 +48,26 @@ static int env_int(const char *name, int dflt)
     return v ? atoi(v) : dflt;
 }
 
+/* One history snapshot: host layout -> file node order (inverse RCM). */
+static int write_snapshot(FILE *fp, const swe_problem *prob, const double *H, int step)
+{
+    const int N = prob->g.Nnodes;
+    double *row = (double *)malloc(sizeof(double) * 4 * (size_t)N);
+    if (!row)
+        return -1;
+    for (int i = 0; i < N; i++) {
+        const int file_node = prob->mapping ? prob->mapping[i] : i;
+        for (int v = 0; v < 4; v++)
+            row[4 * (size_t)file_node + v] = H[swe_svm_lin_id(&prob->layout, i, v)];
+    }
+    const int32_t st = step;
+    const double t = step * prob->g.dt;
+    int ok = fwrite(&st, sizeof(st), 1, fp) == 1 && fwrite(&t, sizeof(t), 1, fp) == 1
+             && fwrite(row, sizeof(double), 4 * (size_t)N, fp) == 4 * (size_t)N;
+    free(row);
+    return ok ? 0 : -1;
+}
+
 static void hip_check(int rc, const char *what)
 {
     if (rc != SWE_HIP_OK) {
@@ -54,7 +81,7 @@ int main(void)
     const double t_start = swe_get_time();
     const char *input = need_env("SWE_INPUT_FILE");
     const int use_netcdf = atoi(need_env("SWE_USE_NETCDF"));
-    (void)atoi(need_env("SWE_NUM_COMPUTE_UNITS"));
+    (void)need_env("SWE_NUM_COMPUTE_UNITS");   /* accepted, one GPU */
     const int nsteps = atoi(need_env("SWE_NSTEPS"));
     const int nattempts = atoi(need_env("SWE_NATTEMPTS"));
     const int use_rcm = atoi(need_env("SWE_USE_RCM"));
@@ -62,6 +89,8 @@ int main(void)
     const char *mode = getenv("SWE_HIP_MODE") ? getenv("SWE_HIP_MODE") : "substep";
     const int exact = env_int("SWE_HIP_EXACT", 0);
     const int members = env_int("SWE_MEMBERS", 1);
+    const char *hist_path = getenv("SWE_OUTPUT_FILE");
+    const int hist_int = env_int("SWE_HIST_INT", 0);
     swe_layout lay = {0, 0, env_int("SWE_SIMD_LENGTH", 4)};
     if (!strcmp(layout_name, "cfdl")) lay.cfdl = 1;
     else if (!strcmp(layout_name, "sfdl")) lay.sfdl = 1;
@@ -116,18 +145,42 @@ int main(void)
     double *H = (double *)malloc(sizeof(double) * 4 * (size_t)p->patch_size);
     if (!H)
         return 1;
+    FILE *hist = NULL;
+    if (hist_path && hist_int > 0) {
+        hist = fopen(hist_path, "wb");
+        if (!hist) {
+            printf("Error opening file %s\n", hist_path);
+            return 2;
+        }
+        const int32_t n32 = prob.g.Nnodes;
+        fwrite(&n32, sizeof(n32), 1, hist);
+        printf("Write history to output file: yes\n\tOutput File Path: %s\n\tHistory Interval "
+               "(timesteps): %d\n", hist_path, hist_int);
+    }
     for (int attempt = 0; attempt < nattempts; attempt++) {
         for (int m = 0; m < members; m++)
             hip_check(swe_hip_set_state(ctx, m, prob.H_init), "swe_hip_set_state");
         hip_check(swe_hip_set_timing(ctx, substep_mode), "swe_hip_set_timing");
         timer.attempt = attempt;
         const double t0 = swe_get_time();
-        if (substep_mode) {
-            for (int i = 1; i <= nsteps; i++)
-                for (int rk = 0; rk < 4; rk++)
-                    hip_check(swe_hip_rk_substep(ctx, rk), "swe_hip_rk_substep");
-        } else {
-            hip_check(swe_hip_rk_steps(ctx, nsteps), "swe_hip_rk_steps");
+        const int chunk = (hist && attempt == 0) ? hist_int : nsteps;
+        for (int done = 0; done < nsteps;) {
+            const int k = nsteps - done < chunk ? nsteps - done : chunk;
+            if (substep_mode) {
+                for (int i = 0; i < k; i++)
+                    for (int rk = 0; rk < 4; rk++)
+                        hip_check(swe_hip_rk_substep(ctx, rk), "swe_hip_rk_substep");
+            } else {
+                hip_check(swe_hip_rk_steps(ctx, k), "swe_hip_rk_steps");
+            }
+            done += k;
+            if (hist && attempt == 0 && done % hist_int == 0) {
+                hip_check(swe_hip_get_state(ctx, 0, H), "swe_hip_get_state");
+                if (write_snapshot(hist, &prob, H, done)) {
+                    fprintf(stderr, "swe_run: history write failed\n");
+                    return 1;
+                }
+            }
         }
         hip_check(swe_hip_synchronize(ctx), "swe_hip_synchronize");
         timer.t_main[attempt] = swe_get_time() - t0;
@@ -152,6 +205,8 @@ int main(void)
             printf("VERIFICATION SUCCEEDED\n");
         fflush(stdout);
     }
+    if (hist)
+        fclose(hist);
     swe_process_profiling(&timer, nsteps, 1, NULL);
     free(H);
     swe_hip_destroy(ctx);

@@ -337,3 +337,19 @@ def case_path(n_nodes: int, seed: int = 1, dt: float | None = None,
     if not os.path.exists(path):
         write_bin(path, make_case(n_nodes, seed=seed, dt=dt, kappa=kappa))
     return path
+
+
+def read_history(path: str):
+    """History file of swe_run (SWE_OUTPUT_FILE / SWE_HIST_INT): returns
+    (steps, times, states[nsnap][N][4]) in the file node order."""
+    b = np.fromfile(path, dtype=np.uint8)
+    N = int(b[:4].view("<i4")[0])
+    rec = 4 + 8 + 32 * N
+    nsnap = (len(b) - 4) // rec
+    steps, times, states = [], [], []
+    for k in range(nsnap):
+        o = 4 + k * rec
+        steps.append(int(b[o:o + 4].view("<i4")[0]))
+        times.append(float(b[o + 4:o + 12].view("<f8")[0]))
+        states.append(b[o + 12:o + rec].view("<f8").reshape(N, 4).copy())
+    return steps, times, np.array(states)

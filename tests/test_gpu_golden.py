@@ -55,3 +55,29 @@ def test_known_answer_100_steps(pkg):
         Hf = ctx.get_state()
     assert prob.verify(Hf, prob.H_100, 1e-9) == 0
     assert (rel_by_var(Hf, prob.H_100, prob) <= 1e-12).all()
+
+
+@pytest.mark.parametrize("mode", ["substep", "steps"])
+def test_driver_history_and_known_answer(pkg, tmp_path, mode):
+    """swe_run (the reference driver on the HIP path): VERIFICATION SUCCEEDED
+    on the reference-made known answer, and its history snapshots equal the
+    reference build's states (natural node order) bit for bit."""
+    import subprocess
+    from conftest import ROOT
+    hist = tmp_path / "hist.bin"
+    env = dict(os.environ, SWE_INPUT_FILE=CASE, SWE_USE_NETCDF="0", SWE_NUM_COMPUTE_UNITS="1",
+               SWE_NSTEPS="100", SWE_NATTEMPTS="3", SWE_USE_RCM="1", SWE_USE_OCL="0",
+               SWE_LAYOUT="sfdl", SWE_HIP_MODE=mode, SWE_HIP_EXACT="1",
+               SWE_OUTPUT_FILE=str(hist), SWE_HIST_INT="10")
+    print(f"case: swe_run {mode}", flush=True)
+    r = subprocess.run([os.path.join(ROOT, "rbf-fd-swe_amd", "swe_run")], env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("VERIFICATION SUCCEEDED") == 3
+    assert "Eval_Rhs" in r.stdout
+    steps, times, states = pkg.mesh.read_history(str(hist))
+    assert steps == list(range(10, 101, 10))
+    g = np.load(REFV, allow_pickle=False)
+    assert np.array_equal(states[0], g["H10"])
+    raw = pkg.mesh.read_bin(CASE)
+    assert np.array_equal(states[-1], raw.H_final)
