@@ -57,6 +57,8 @@ def parse_args():
     ap.add_argument("--member-block", type=int, default=0)
     ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane"])
     ap.add_argument("--prefetch", type=int, default=0)
+    ap.add_argument("--order", default="host", choices=["host", "morton"])
+    ap.add_argument("--tiled", action="store_true")
     ap.add_argument("--layout", default="cfdl")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -228,7 +230,7 @@ def main():
           flush=True)
     ctx = pkg.runtime.HipContext(prob, members=args.members, exact=args.exact,
                                  member_block=args.member_block, kernel=args.kernel,
-                                 prefetch=args.prefetch)
+                                 prefetch=args.prefetch, order=args.order, tiled=args.tiled)
     for m in range(args.members):
         ctx.set_state(prob.H_init, m)
     stream = torch.cuda.ExternalStream(ctx.stream())
@@ -287,7 +289,7 @@ def main():
                                f"{args.members} member(s), fused stage kernels + HIP graphs",
                    "nodes": prob.n_nodes, "members": args.members, "layout": args.layout,
                    "member_block": args.member_block, "kernel": args.kernel,
-                   "prefetch": args.prefetch,
+                   "prefetch": args.prefetch, "order": args.order, "tiled": args.tiled,
                    "exact": bool(args.exact), "dt": prob.dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
