@@ -59,6 +59,7 @@ def parse_args():
     ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--order", default="host", choices=["host", "morton"])
     ap.add_argument("--tiled", action="store_true")
+    ap.add_argument("--slot-block", type=int, default=0, choices=[0, 16])
     ap.add_argument("--layout", default="cfdl")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -230,7 +231,8 @@ def main():
           flush=True)
     ctx = pkg.runtime.HipContext(prob, members=args.members, exact=args.exact,
                                  member_block=args.member_block, kernel=args.kernel,
-                                 prefetch=args.prefetch, order=args.order, tiled=args.tiled)
+                                 prefetch=args.prefetch, order=args.order, tiled=args.tiled,
+                                 slot_block=args.slot_block)
     for m in range(args.members):
         ctx.set_state(prob.H_init, m)
     stream = torch.cuda.ExternalStream(ctx.stream())
@@ -290,6 +292,7 @@ def main():
                    "nodes": prob.n_nodes, "members": args.members, "layout": args.layout,
                    "member_block": args.member_block, "kernel": args.kernel,
                    "prefetch": args.prefetch, "order": args.order, "tiled": args.tiled,
+                   "slot_block": args.slot_block,
                    "exact": bool(args.exact), "dt": prob.dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -104,6 +104,9 @@ typedef struct swe_hip_options {
                                    with share: the mesh's order is used)    */
     int tiled;                  /* 1: LDS-staged ensemble kernel (members
                                    >= 2, member_block 4, 8 or 16)           */
+    int slot_block;             /* stencil arrays: 0 slot-major [nnbr][n];
+                                   16 node blocks [n/16][nnbr][16] (ignored
+                                   with share)                              */
 } swe_hip_options;
 int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip_ctx **out);
 

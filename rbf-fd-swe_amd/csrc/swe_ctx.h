@@ -46,6 +46,7 @@ struct SweMesh {
     double *d_wts = nullptr, *d_geo = nullptr;
     long long bytes = 0;
     int order = SWE_HIP_ORDER_HOST;
+    int sblk = 0;                 // stencil layout: 0 slot-major, 16 node blocks
     std::vector<int> perm;        // device node -> host node (empty: identity)
     std::vector<int> h_nbr;       // host copy of nbr, device order, [nnbr][n]
 };
@@ -68,6 +69,7 @@ struct swe_hip_ctx {
     swe::HostLayout lay{1, 0, 1};
     int *d_nbr = nullptr;
     double *d_wts = nullptr, *d_geo = nullptr;
+    int sblk = 0;
     int B = 1, MB = 1;
     int kernel = SWE_HIP_KERNEL_QUAD;   // single-state mapping
     int prefetch = 0;                   // stencil slots per load batch (0, 4, 8)
@@ -105,6 +107,7 @@ inline DevMesh mesh_of(const swe_hip_ctx *c)
     m.n = c->n;
     m.ld = c->ld;
     m.nnbr = c->nnbr;
+    m.sblk = c->sblk;
     m.gh0 = c->gh0;
     return m;
 }

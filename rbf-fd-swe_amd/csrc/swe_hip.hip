@@ -289,7 +289,7 @@ uint64_t morton3(double x, double y, double z)
     return spread(q(x)) | spread(q(y)) << 1 | spread(q(z)) << 2;
 }
 
-int build_mesh(const swe_hip_static *s, int device, int order, SweMesh **out)
+int build_mesh(const swe_hip_static *s, int device, int order, int sblk, SweMesh **out)
 {
     *out = nullptr;
     const int n = s->compute_size;
@@ -313,9 +313,17 @@ int build_mesh(const swe_hip_static *s, int device, int order, SweMesh **out)
     }
     auto host_of = [&](int d) { return perm.empty() ? d : perm[d]; };
     auto dev_of = [&](int h) { return inv.empty() ? h : inv[h]; };
-    // Repack into the device layouts, checking every stencil id.
+    // Repack into the device layouts, checking every stencil id.  nbr keeps
+    // the slot-major form [nnbr][n] (the tile builder reads it); the device
+    // stencil arrays are slot-major or in node blocks of 16.
+    const size_t nround = sblk ? ((size_t)n + 15) / 16 * 16 : (size_t)n;
+    auto pos = [&](int j, int dnode) -> size_t {
+        return sblk ? (((size_t)(dnode >> 4) * W + j) << 4) + (dnode & 15)
+                    : (size_t)j * n + dnode;
+    };
     std::vector<int> nbr((size_t)W * n);
-    std::vector<double> wts((size_t)W * n * 4);
+    std::vector<int> dnbr(sblk ? (size_t)W * nround : 0);
+    std::vector<double> wts((size_t)W * nround * 4);
     std::vector<double> geo((size_t)GEO_N * n);
     for (int dnode = 0; dnode < n; ++dnode) {
         const int i = host_of(dnode);
@@ -326,7 +334,9 @@ int build_mesh(const swe_hip_static *s, int device, int order, SweMesh **out)
                 return fail(SWE_HIP_ERANGE, "idx[" + std::to_string(d) + "] = " + std::to_string(q)
                                                 + " outside [0, " + std::to_string(n) + ")");
             nbr[(size_t)j * n + dnode] = dev_of(q);
-            double *w = &wts[((size_t)j * n + dnode) * 4];
+            if (sblk)
+                dnbr[pos(j, dnode)] = dev_of(q);
+            double *w = &wts[pos(j, dnode) * 4];
             w[0] = s->Dx[d];
             w[1] = s->Dy[d];
             w[2] = s->Dz[d];
@@ -345,8 +355,18 @@ int build_mesh(const swe_hip_static *s, int device, int order, SweMesh **out)
             geo[(size_t)(G_GG + c3) * n + dnode] = s->gradghm[v];
         }
     }
+    if (sblk) {   // rows of the last block beyond n repeat the last node
+        for (int dnode = n; dnode < (int)nround; ++dnode)
+            for (int j = 0; j < W; ++j) {
+                dnbr[pos(j, dnode)] = dnbr[pos(j, n - 1)];
+                for (int c4 = 0; c4 < 4; ++c4)
+                    wts[pos(j, dnode) * 4 + c4] = wts[pos(j, n - 1) * 4 + c4];
+            }
+    }
+    const std::vector<int> &upload_nbr = sblk ? dnbr : nbr;
     SweMesh *m = new SweMesh();
     m->order = order;
+    m->sblk = sblk;
     m->dev = device;
     m->n = n;
     m->ld = n;
@@ -355,11 +375,13 @@ int build_mesh(const swe_hip_static *s, int device, int order, SweMesh **out)
     m->dt = s->dt;
     m->lay = lay;
     int rc = 0;
-    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_nbr, sizeof(int) * nbr.size());
+    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_nbr, sizeof(int) * upload_nbr.size());
     if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_wts, sizeof(double) * wts.size());
     if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_geo, sizeof(double) * geo.size());
     hipError_t e = hipSuccess;
-    if (!rc) e = hipMemcpy(m->d_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
+    if (!rc)
+        e = hipMemcpy(m->d_nbr, upload_nbr.data(), sizeof(int) * upload_nbr.size(),
+                      hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess)
         e = hipMemcpy(m->d_wts, wts.data(), sizeof(double) * wts.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess)
@@ -474,6 +496,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "prefetch must be 0, 4 or 8");
     if (o->node_order != SWE_HIP_ORDER_HOST && o->node_order != SWE_HIP_ORDER_MORTON)
         return fail(SWE_HIP_EINVAL, "node_order must be SWE_HIP_ORDER_HOST or _MORTON");
+    if (o->slot_block != 0 && o->slot_block != 16)
+        return fail(SWE_HIP_EINVAL, "slot_block must be 0 or 16");
     if (!o->share) {
         const int rc = validate_static(s);
         if (rc)
@@ -498,7 +522,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         m = o->share->mesh;
         m->refs++;
     } else {
-        int rc = build_mesh(s, device, o->node_order, &m);
+        int rc = build_mesh(s, device, o->node_order, o->slot_block, &m);
         if (rc)
             return rc;
     }
@@ -512,6 +536,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->dt = m->dt;
     c->lay = m->lay;
     c->d_nbr = m->d_nbr;
+    c->sblk = m->sblk;
     c->d_wts = m->d_wts;
     c->d_geo = m->d_geo;
     c->B = o->members;
@@ -558,7 +583,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
 int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int device,
                    swe_hip_ctx **out)
 {
-    swe_hip_options o{members, flags, device, 0, nullptr, 0, 0, 0, 0};
+    swe_hip_options o{members, flags, device, 0, nullptr, 0, 0, 0, 0, 0};
     return swe_hip_create_ex(s, &o, out);
 }
 
@@ -574,7 +599,7 @@ int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, in
     for (int mb = 64; mb >= 2; mb >>= 1) {
         if (members % mb)
             continue;
-        swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, 0};
+        swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, 0, 0};
         swe_hip_ctx *c = nullptr;
         int rc = swe_hip_create_ex(nullptr, &o, &c);
         if (rc)

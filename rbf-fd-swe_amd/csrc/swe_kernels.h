@@ -41,10 +41,25 @@ struct DevMesh {
     const double *__restrict__ wts;
     const double *__restrict__ geo;
     int n;        // nodes
-    int ld;       // leading dimension of nbr/wts/geo (>= n)
+    int ld;       // leading dimension of geo (and of nbr/wts when sblk == 0)
     int nnbr;     // stencil width (loop bound)
+    int sblk;     // 0: nbr/wts slot-major [nnbr][ld]; 16: node blocks [n/16][nnbr][16]
     double gh0;
 };
+
+// Position of (slot 0, node) in nbr (and /4 in wts), and the distance
+// between consecutive slots of one node, for both stencil layouts.
+template <int NNBR>
+__device__ __forceinline__ long slot_base(const DevMesh &m, int node)
+{
+    const long W = NNBR > 0 ? NNBR : m.nnbr;
+    return m.sblk ? ((long)(node >> 4) * W << 4) + (node & 15) : (long)node;
+}
+
+__device__ __forceinline__ long slot_stride(const DevMesh &m)
+{
+    return m.sblk ? 16L : (long)m.ld;
+}
 
 struct DevStage {
     const double *src;  // gather source: H (stage 0) or K
@@ -242,9 +257,10 @@ __global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
     const bool live = node_raw < m.n;
     const int node = live ? node_raw : m.n - 1;  // keep quads whole for DPP
     const int nn = NNBR > 0 ? NNBR : m.nnbr;
-    const long ld = m.ld;
-    const int *__restrict__ nb = m.nbr + node;
-    const double *__restrict__ wp = m.wts + 4L * node;
+    const long ld = slot_stride(m);
+    const long sb = slot_base<NNBR>(m, node);
+    const int *__restrict__ nb = m.nbr + sb;
+    const double *__restrict__ wp = m.wts + 4L * sb;
     const double *__restrict__ src = st.src;
 
     double sx = 0.0, sy = 0.0, sz = 0.0, sl = 0.0;
@@ -297,9 +313,10 @@ __global__ __launch_bounds__(256) void k_stage_lane(DevMesh m, DevStage st)
     const bool live = t < m.n;
     const int node = live ? t : m.n - 1;
     const int nn = NNBR > 0 ? NNBR : m.nnbr;
-    const long ld = m.ld;
-    const int *__restrict__ nb = m.nbr + node;
-    const double *__restrict__ wp = m.wts + 4L * node;
+    const long ld = slot_stride(m);
+    const long sb = slot_base<NNBR>(m, node);
+    const int *__restrict__ nb = m.nbr + sb;
+    const double *__restrict__ wp = m.wts + 4L * sb;
     const double *__restrict__ src = st.src;
 
     Sums S = {};
@@ -361,10 +378,11 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
     const bool live = node_raw < m.n;
     const int node = live ? node_raw : m.n - 1;
     const int nn = NNBR > 0 ? NNBR : m.nnbr;
-    const long ld = m.ld;
+    const long ld = slot_stride(m);
+    const long sb = slot_base<NNBR>(m, node);
     const long B = st.members;
-    const int *__restrict__ nb = m.nbr + node;
-    const double *__restrict__ wp = m.wts + 4L * node;
+    const int *__restrict__ nb = m.nbr + sb;
+    const double *__restrict__ wp = m.wts + 4L * sb;
     const double *__restrict__ src = st.src + mem;
 
     Sums S = {};
@@ -455,12 +473,13 @@ __global__ __launch_bounds__(512) void k_stage_tiled(DevMesh m, DevTiles tl, Dev
         return;   // no barrier follows
     const int node = t0 + nl;
     const int nn = NNBR > 0 ? NNBR : m.nnbr;
-    const long ld = m.ld;
+    const long ld = slot_stride(m);
+    const long lxs = m.ld;   // tile-local ids are slot-major [nnbr][n]
     const unsigned short *__restrict__ lx = tl.lidx + node;
-    const double *__restrict__ wp = m.wts + 4L * node;
+    const double *__restrict__ wp = m.wts + 4L * slot_base<NNBR>(m, node);
     Sums S = {};
     auto slot = [&](int j) {
-        const double *sp = rows + (int)lx[j * ld] * 4 * MB + ml;
+        const double *sp = rows + (int)lx[j * lxs] * 4 * MB + ml;
         const double su = sp[0], sv = sp[MB], sw = sp[2 * MB], sh = sp[3 * MB];
         const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
         const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);

@@ -43,7 +43,8 @@ class SweHipStatic(C.Structure):
 class SweHipOptions(C.Structure):
     _fields_ = [("members", C.c_int), ("flags", C.c_uint), ("device", C.c_int),
                 ("member_block", C.c_int), ("share", C.c_void_p), ("kernel", C.c_int),
-                ("prefetch", C.c_int), ("node_order", C.c_int), ("tiled", C.c_int)]
+                ("prefetch", C.c_int), ("node_order", C.c_int), ("tiled", C.c_int),
+                ("slot_block", C.c_int)]
 
 
 class SweHipTimers(C.Structure):
@@ -142,13 +143,14 @@ class HipContext:
     def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0,
                  member_block: int = 0, share: "HipContext | None" = None,
                  kernel: str = "auto", prefetch: int = 0, order: str = "host",
-                 tiled: bool = False):
+                 tiled: bool = False, slot_block: int = 0):
         self.prob = prob
         self.members = int(members)
         self.exact = bool(exact)
         opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
                              int(member_block), share.handle if share is not None else None,
-                             KERNELS[kernel], int(prefetch), ORDERS[order], int(bool(tiled)))
+                             KERNELS[kernel], int(prefetch), ORDERS[order], int(bool(tiled)),
+                             int(slot_block))
         h = C.c_void_p()
         if share is not None:
             rc = lib().swe_hip_create_ex(None, C.byref(opts), C.byref(h))

@@ -144,13 +144,31 @@ def test_tiled_fast_composition_and_active(pkg):
         assert np.array_equal(solo.get_state(), got15)
 
 
-@pytest.mark.parametrize("kernel,pf", [("quad", 0), ("quad", 8), ("lane", 0), ("lane", 4)])
-def test_single_state_variants_exact(pkg, pyoracle, kernel, pf):
+@pytest.mark.parametrize("kernel,pf,sblk", [("quad", 0, 0), ("quad", 8, 0), ("lane", 0, 0),
+                                           ("lane", 4, 0), ("quad", 0, 16), ("lane", 4, 16)])
+def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk):
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
-    print(f"case: single-state {kernel} prefetch={pf} morton exact", flush=True)
+    print(f"case: single-state {kernel} prefetch={pf} slot_block={sblk} morton exact", flush=True)
     want = pyoracle.Oracle(prob).run(prob.H_init, 3)
     with pkg.runtime.HipContext(prob, exact=True, kernel=kernel, prefetch=pf,
-                                order="morton") as ctx:
+                                order="morton", slot_block=sblk) as ctx:
         ctx.set_state(prob.H_init)
         ctx.rk_steps(3)
         assert np.array_equal(ctx.get_state(), want)
+
+
+@pytest.mark.parametrize("tiled,mb", [(False, 64), (False, 8), (True, 8)])
+def test_blocked_stencil_layout_members_exact(pkg, pyoracle, tiled, mb):
+    """Node-blocked stencil arrays (slot_block=16, N=642 is not a multiple
+    of 16) give the oracle's bits for the ensemble kernels too."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(642, seed=1), use_rcm=True, layout="cfdl")
+    print(f"case: slot_block=16 members tiled={tiled} mb={mb}", flush=True)
+    H0 = _members(pkg, prob, 64)
+    orc = pyoracle.Oracle(prob)
+    with pkg.runtime.HipContext(prob, members=64, member_block=mb, exact=True, tiled=tiled,
+                                order="morton" if tiled else "host", slot_block=16) as ctx:
+        for m in range(64):
+            ctx.set_state(H0[m], m)
+        ctx.rk_steps(3)
+        for m in (0, 33, 63):
+            assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 3)), m
