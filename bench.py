@@ -55,7 +55,7 @@ def parse_args():
     ap.add_argument("--nodes", type=int, default=40962)
     ap.add_argument("--members", type=int, default=1)
     ap.add_argument("--member-block", type=int, default=0)
-    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw"])
     ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--order", default="host", choices=["host", "morton"])
     ap.add_argument("--tiled", action="store_true")

@@ -89,6 +89,8 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
 #define SWE_HIP_KERNEL_AUTO 0
 #define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
 #define SWE_HIP_KERNEL_LANE 2   /* single state: 1 lane per node             */
+#define SWE_HIP_KERNEL_QUADW 3  /* single state: 4 lanes per node, weights
+                                   loaded once per quad and shared by DPP    */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 typedef struct swe_hip_options {

@@ -30,6 +30,19 @@ void launch_lane(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
         k_stage_lane<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
 }
 
+template <int STAGE, bool EX, int NN>
+void launch_quadw(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    st.nblk = blocks_x(c, 4L * c->n);
+    const dim3 g(st.nblk), b(kBlock);
+    if (NN > 0 && c->prefetch == 8)
+        k_stage_quadw<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
+    else if (NN > 0 && c->prefetch == 4)
+        k_stage_quadw<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_quadw<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
+}
+
 }  // namespace
 
 template <int STAGE, bool EX, int NN>
@@ -38,6 +51,8 @@ void family_single(const swe_hip_ctx *c, const DevStage &st)
     const DevMesh m = mesh_of(c);
     if (c->kernel == SWE_HIP_KERNEL_LANE)
         launch_lane<STAGE, EX, NN>(c, m, st);
+    else if (c->kernel == SWE_HIP_KERNEL_QUADW)
+        launch_quadw<STAGE, EX, NN>(c, m, st);
     else
         launch_quad<STAGE, EX, NN>(c, m, st);
 }

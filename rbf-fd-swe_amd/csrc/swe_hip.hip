@@ -541,7 +541,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->d_geo = m->d_geo;
     c->B = o->members;
     c->MB = mb;
-    c->kernel = o->kernel == SWE_HIP_KERNEL_LANE ? SWE_HIP_KERNEL_LANE : SWE_HIP_KERNEL_QUAD;
+    c->kernel = (o->kernel == SWE_HIP_KERNEL_LANE || o->kernel == SWE_HIP_KERNEL_QUADW)
+                    ? o->kernel : SWE_HIP_KERNEL_QUAD;
     c->prefetch = o->prefetch;
     c->flags = o->flags;
     int rc = 0;

@@ -298,6 +298,67 @@ __global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
         stage_update<STAGE, EXACT>(st, 4L * node + k, Fk);
 }
 
+struct QuadWSlot {
+    double w;
+    double s;
+};
+
+// Single state, quad mapping with shared weight loads: lane k of the quad
+// loads only weight k (Dx, Dy, Dz or L) of each slot -- one 8-byte load per
+// lane, a wave's 16 nodes x 32 B read once instead of four times -- and the
+// quad exchanges the four weights by DPP.  Same sums, same order as
+// k_stage_quad.
+template <int STAGE, bool EXACT, int NNBR, int PF>
+__global__ __launch_bounds__(256) void k_stage_quadw(DevMesh m, DevStage st)
+{
+    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int node_raw = t >> 2;
+    const int k = t & 3;
+    const bool live = node_raw < m.n;
+    const int node = live ? node_raw : m.n - 1;
+    const int nn = NNBR > 0 ? NNBR : m.nnbr;
+    const long ld = slot_stride(m);
+    const long sb = slot_base<NNBR>(m, node);
+    const int *__restrict__ nb = m.nbr + sb;
+    const double *__restrict__ wp = m.wts + 4L * sb + k;
+    const double *__restrict__ src = st.src;
+
+    double sx = 0.0, sy = 0.0, sz = 0.0, sl = 0.0;
+    auto load = [&](int j, int q) {
+        QuadWSlot r;
+        r.w = wp[4L * j * ld];
+        r.s = src[4L * q + k];
+        return r;
+    };
+    auto acc = [&](const QuadWSlot &r) {
+        const double wx = quad_bcast<0>(r.w), wy = quad_bcast<1>(r.w);
+        const double wz = quad_bcast<2>(r.w), wl = quad_bcast<3>(r.w);
+        sx = madd<EXACT>(sx, wx, r.s);
+        sy = madd<EXACT>(sy, wy, r.s);
+        sz = madd<EXACT>(sz, wz, r.s);
+        sl = madd<EXACT>(sl, wl, r.s);
+    };
+    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (QuadWSlot *)nullptr);
+    Sums S;
+    S.dudx = quad_bcast<0>(sx); S.dudy = quad_bcast<0>(sy);
+    S.dudz = quad_bcast<0>(sz); S.lu = quad_bcast<0>(sl);
+    S.dvdx = quad_bcast<1>(sx); S.dvdy = quad_bcast<1>(sy);
+    S.dvdz = quad_bcast<1>(sz); S.lv = quad_bcast<1>(sl);
+    S.dwdx = quad_bcast<2>(sx); S.dwdy = quad_bcast<2>(sy);
+    S.dwdz = quad_bcast<2>(sz); S.lw = quad_bcast<2>(sl);
+    S.dhdx = quad_bcast<3>(sx); S.dhdy = quad_bcast<3>(sy);
+    S.dhdz = quad_bcast<3>(sz); S.lh = quad_bcast<3>(sl);
+
+    const double2 s01 = *reinterpret_cast<const double2 *>(src + 4L * node);
+    const double2 s23 = *reinterpret_cast<const double2 *>(src + 4L * node + 2);
+    const Geo g = load_geo(m.geo, m.ld, node);
+    double F[4];
+    rhs_point(S, s01.x, s01.y, s23.x, s23.y, g, m.gh0, F);
+    const double Fk = k == 0 ? F[0] : k == 1 ? F[1] : k == 2 ? F[2] : F[3];
+    if (live)
+        stage_update<STAGE, EXACT>(st, 4L * node + k, Fk);
+}
+
 struct LaneSlot {
     double2 w01, w23, s01, s23;
 };

@@ -145,7 +145,8 @@ def test_tiled_fast_composition_and_active(pkg):
 
 
 @pytest.mark.parametrize("kernel,pf,sblk", [("quad", 0, 0), ("quad", 8, 0), ("lane", 0, 0),
-                                           ("lane", 4, 0), ("quad", 0, 16), ("lane", 4, 16)])
+                                           ("lane", 4, 0), ("quad", 0, 16), ("lane", 4, 16),
+                                           ("quadw", 0, 0), ("quadw", 4, 16)])
 def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk):
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
     print(f"case: single-state {kernel} prefetch={pf} slot_block={sblk} morton exact", flush=True)
