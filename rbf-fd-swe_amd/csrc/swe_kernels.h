@@ -447,31 +447,70 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
     const double *__restrict__ src = st.src + mem;
 
     Sums S = {};
-    auto load = [&](int j, int q) {
-        if constexpr (MB == 64)
-            q = __builtin_amdgcn_readfirstlane(q);
-        MemberSlot r;
-        r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
-        r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
-        const double *sq = src + 4L * q * B;
-        r.su = sq[0];
-        r.sv = sq[B];
-        r.sw = sq[2 * B];
-        r.sh = sq[3 * B];
-        return r;
-    };
-    auto acc = [&](const MemberSlot &r) {
-        const double su = r.su, sv = r.sv, sw = r.sw, sh = r.sh;
-        S.dudx = madd<EXACT>(S.dudx, r.w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, r.w01.x, sv);
-        S.dwdx = madd<EXACT>(S.dwdx, r.w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, r.w01.x, sh);
-        S.dudy = madd<EXACT>(S.dudy, r.w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, r.w01.y, sv);
-        S.dwdy = madd<EXACT>(S.dwdy, r.w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, r.w01.y, sh);
-        S.dudz = madd<EXACT>(S.dudz, r.w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, r.w23.x, sv);
-        S.dwdz = madd<EXACT>(S.dwdz, r.w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, r.w23.x, sh);
-        S.lu = madd<EXACT>(S.lu, r.w23.y, su); S.lv = madd<EXACT>(S.lv, r.w23.y, sv);
-        S.lw = madd<EXACT>(S.lw, r.w23.y, sw); S.lh = madd<EXACT>(S.lh, r.w23.y, sh);
-    };
-    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (MemberSlot *)nullptr);
+    if constexpr (MB == 64 && !EXACT && NNBR > 0 && NNBR <= 32) {
+        // One node per wave: lane j fetches the id of slot j and lanes 2j,
+        // 2j+1 fetch its (Dx, Dy) and (Dz, L), one vector load each, up
+        // front.  The weights go to a wave-private LDS row and are read back
+        // per slot with uniform-address (broadcast) LDS loads; the id moves
+        // to a scalar register with v_readlane.  No scalar-memory round trip
+        // sits between the neighbour gathers, so several slots of gathers
+        // stay in flight.
+        __shared__ double2 wrow[4][64];
+        const int lane = (int)threadIdx.x & 63;
+        const int wid = (int)threadIdx.x >> 6;
+        const int jq = lane < NNBR ? lane : 0;
+        const int jw = (lane >> 1) < NNBR ? (lane >> 1) : 0;
+        const int qv = nb[jq * ld];
+        wrow[wid][lane] = *reinterpret_cast<const double2 *>(wp + 4L * jw * ld + 2 * (lane & 1));
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < NNBR; ++j) {
+            const int q = __builtin_amdgcn_readlane(qv, j);
+            const double2 w01 = wrow[wid][2 * j];
+            const double2 w23 = wrow[wid][2 * j + 1];
+            const double *sq = src + 4L * q * B;
+            const double su = sq[0], sv = sq[B], sw = sq[2 * B], sh = sq[3 * B];
+            S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
+            S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
+            S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
+            S.dwdy = madd<EXACT>(S.dwdy, w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, w01.y, sh);
+            S.dudz = madd<EXACT>(S.dudz, w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, w23.x, sv);
+            S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
+            S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
+            S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
+            if constexpr (PF > 0) {
+                if (j % PF == PF - 1)
+                    __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    } else {
+        auto load = [&](int j, int q) {
+            if constexpr (MB == 64)
+                q = __builtin_amdgcn_readfirstlane(q);
+            MemberSlot r;
+            r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+            r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+            const double *sq = src + 4L * q * B;
+            r.su = sq[0];
+            r.sv = sq[B];
+            r.sw = sq[2 * B];
+            r.sh = sq[3 * B];
+            return r;
+        };
+        auto acc = [&](const MemberSlot &r) {
+            const double su = r.su, sv = r.sv, sw = r.sw, sh = r.sh;
+            S.dudx = madd<EXACT>(S.dudx, r.w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, r.w01.x, sv);
+            S.dwdx = madd<EXACT>(S.dwdx, r.w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, r.w01.x, sh);
+            S.dudy = madd<EXACT>(S.dudy, r.w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, r.w01.y, sv);
+            S.dwdy = madd<EXACT>(S.dwdy, r.w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, r.w01.y, sh);
+            S.dudz = madd<EXACT>(S.dudz, r.w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, r.w23.x, sv);
+            S.dwdz = madd<EXACT>(S.dwdz, r.w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, r.w23.x, sh);
+            S.lu = madd<EXACT>(S.lu, r.w23.y, su); S.lv = madd<EXACT>(S.lv, r.w23.y, sv);
+            S.lw = madd<EXACT>(S.lw, r.w23.y, sw); S.lh = madd<EXACT>(S.lh, r.w23.y, sh);
+        };
+        stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (MemberSlot *)nullptr);
+    }
     const double *own = src + 4L * node * B;
     const double u = own[0], v = own[B], w = own[2 * B], h = own[3 * B];
     const Geo g = load_geo(m.geo, m.ld, node);

@@ -28,11 +28,21 @@ def test_code_object_has_no_forbidden_instructions():
         co = os.path.join(d, "gfx950.co")
         subprocess.run([f"{LLVM}/llvm-objcopy", f"--dump-section=.hip_fatbin={fat}", so,
                         os.path.join(d, "junk.so")], check=True)
-        subprocess.run([f"{LLVM}/clang-offload-bundler", "--unbundle", "--type=o",
-                        "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={fat}",
-                        f"--output={co}"], check=True)
-        asm = subprocess.run([f"{LLVM}/llvm-objdump", "-d", co], capture_output=True,
-                             text=True, check=True).stdout
+        # one offload bundle per translation unit, concatenated
+        data = open(fat, "rb").read()
+        magic = b"__CLANG_OFFLOAD_BUNDLE__"
+        starts = [i for i in range(len(data)) if data.startswith(magic, i)]
+        assert starts, "no offload bundle found"
+        asm = ""
+        for k, (a, b) in enumerate(zip(starts, starts[1:] + [len(data)])):
+            part = os.path.join(d, f"part{k}.bin")
+            with open(part, "wb") as fp:
+                fp.write(data[a:b])
+            subprocess.run([f"{LLVM}/clang-offload-bundler", "--unbundle", "--type=o",
+                            "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={part}",
+                            f"--output={co}"], check=True)
+            asm += subprocess.run([f"{LLVM}/llvm-objdump", "-d", co], capture_output=True,
+                                  text=True, check=True).stdout
     finally:
         shutil.rmtree(d, ignore_errors=True)
     assert "v_fmac_f64" in asm or "v_fma_f64" in asm       # the disassembly is real
