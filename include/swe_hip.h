@@ -86,11 +86,12 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
  * share, when not NULL, is a context of the same mesh whose device copy of
  * the static data is reused (s may then be NULL): the static data stay
  * resident until the last context using them is destroyed. */
-#define SWE_HIP_KERNEL_AUTO 0
+#define SWE_HIP_KERNEL_AUTO 0   /* single state: QUADW                       */
 #define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
 #define SWE_HIP_KERNEL_LANE 2   /* single state: 1 lane per node             */
-#define SWE_HIP_KERNEL_QUADW 3  /* single state: 4 lanes per node, weights
-                                   loaded once per quad and shared by DPP    */
+#define SWE_HIP_KERNEL_QUADW 3  /* single state: 4 lanes per node, weights,
+                                   ids, geometry loaded once per quad and
+                                   shared by DPP                             */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 typedef struct swe_hip_options {

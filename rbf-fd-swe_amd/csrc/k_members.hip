@@ -7,6 +7,18 @@ namespace {
 template <int STAGE, bool EX, int MBv, int NN>
 void launch_members(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
+    if constexpr (MBv == 64 && NN > 0) {
+        // Member pairs: 32 lanes per node.
+        st.nblk = blocks_x(c, (long)c->n * 32);
+        const dim3 g(st.nblk, c->B / 64), b(kBlock);
+        if (c->prefetch == 8)
+            k_stage_mpair<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
+        else if (c->prefetch == 4)
+            k_stage_mpair<STAGE, EX, NN, 2><<<g, b, 0, c->stream>>>(m, st);
+        else
+            k_stage_mpair<STAGE, EX, NN, 3><<<g, b, 0, c->stream>>>(m, st);
+        return;
+    }
     st.nblk = blocks_x(c, (long)c->n * MBv);
     const dim3 g(st.nblk, c->B / MBv), b(kBlock);
     // Exact sums need the pipelined loop: left to the compiler, all the

@@ -4,19 +4,36 @@
 namespace swe {
 namespace {
 
-template <int STAGE, bool EX, int MBv, int NN>
-void launch_tiled(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+DevTiles tiles_of(const swe_hip_ctx *c, int mb)
 {
     DevTiles tl;
     tl.tile_start = c->tiles.d_tile_start;
     tl.ustart = c->tiles.d_ustart;
     tl.ulist = c->tiles.d_ulist;
     tl.lidx = c->tiles.d_lidx;
+    tl.lidx_node = c->tiles.d_lidx_node;
     tl.ntiles = c->tiles.ntiles;
-    tl.nmb = c->B / MBv;
+    tl.nmb = c->B / mb;
+    return tl;
+}
+
+template <int STAGE, bool EX, int MBv, int NN>
+void launch_tiled(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    const DevTiles tl = tiles_of(c, MBv);
     st.nblk = (int)(((long)tl.ntiles * tl.nmb + 7) / 8 * 8);
     const size_t lds = sizeof(double) * 4 * MBv * (size_t)c->tiles.umax;
     k_stage_tiled<STAGE, EX, MBv, NN><<<dim3(st.nblk), dim3(kTiledBlock), lds, c->stream>>>(m, tl, st);
+}
+
+// Member lanes over LDS tiles (member block 32, 31-point stencils).
+template <int STAGE, bool EX>
+void launch_tmem(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    const DevTiles tl = tiles_of(c, 32);
+    st.nblk = (int)(((long)tl.ntiles * tl.nmb + 7) / 8 * 8);
+    const size_t lds = sizeof(double) * 4 * 32 * (size_t)c->tiles.umax;
+    k_stage_tmem<STAGE, EX, 31><<<dim3(st.nblk), dim3(256), lds, c->stream>>>(m, tl, st);
 }
 
 }  // namespace
@@ -37,6 +54,14 @@ namespace {
 template <int STAGE>
 void by_flags(const swe_hip_ctx *c, const DevStage &st)
 {
+    if (c->MB == 32) {   // set up only for 31-point stencils
+        const DevMesh m = mesh_of(c);
+        if (exact(c))
+            launch_tmem<STAGE, true>(c, m, st);
+        else
+            launch_tmem<STAGE, false>(c, m, st);
+        return;
+    }
     const bool nn31 = c->nnbr == 31;
     if (exact(c)) {
         if (nn31) family_tiled<STAGE, true, 31>(c, st);

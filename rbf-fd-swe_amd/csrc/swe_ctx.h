@@ -57,6 +57,7 @@ struct SweTiles {
     int ntiles = 0, umax = 0, tmax = 0;
     int *d_tile_start = nullptr, *d_ustart = nullptr, *d_ulist = nullptr;
     unsigned short *d_lidx = nullptr;
+    unsigned short *d_lidx_node = nullptr;   // [n][32] node-major copy of d_lidx
 };
 
 struct swe_hip_ctx {

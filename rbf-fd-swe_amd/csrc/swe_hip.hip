@@ -219,7 +219,7 @@ void free_ctx(swe_hip_ctx *c)
     if (c->graphN) (void)hipGraphExecDestroy(c->graphN);
     void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
-                    c->tiles.d_lidx};
+                    c->tiles.d_lidx, c->tiles.d_lidx_node};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -405,7 +405,7 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
     const int n = c->n, W = c->nnbr;
     const std::vector<int> &nbr = c->mesh->h_nbr;
     std::vector<int> tstart{0}, ustart{0}, ulist;
-    std::vector<unsigned short> lidx((size_t)W * n);
+    std::vector<unsigned short> lidx((size_t)W * n), lidx_node((size_t)32 * n, 0);
     std::vector<int> mark(n, -1), pos(n, 0);
     std::vector<int> cur;
     int tile = 0, node = 0;
@@ -435,8 +435,12 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
         for (int k = 0; k < (int)cur.size(); ++k)
             pos[cur[k]] = k;
         for (int i = first; i < node; ++i)
-            for (int j = 0; j < W; ++j)
-                lidx[(size_t)j * n + i] = (unsigned short)pos[nbr[(size_t)j * n + i]];
+            for (int j = 0; j < W; ++j) {
+                const unsigned short l = (unsigned short)pos[nbr[(size_t)j * n + i]];
+                lidx[(size_t)j * n + i] = l;
+                if (j < 32)
+                    lidx_node[(size_t)32 * i + j] = l;
+            }
         ulist.insert(ulist.end(), cur.begin(), cur.end());
         tstart.push_back(node);
         ustart.push_back((int)ulist.size());
@@ -451,11 +455,13 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_ustart, sizeof(int) * ustart.size());
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_ulist, sizeof(int) * ulist.size());
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_lidx, sizeof(unsigned short) * lidx.size());
+    if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_lidx_node, sizeof(unsigned short) * lidx_node.size());
     hipError_t e = hipSuccess;
     if (!rc) e = hipMemcpy(t.d_tile_start, tstart.data(), sizeof(int) * tstart.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) e = hipMemcpy(t.d_ustart, ustart.data(), sizeof(int) * ustart.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) e = hipMemcpy(t.d_ulist, ulist.data(), sizeof(int) * ulist.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) e = hipMemcpy(t.d_lidx, lidx.data(), sizeof(unsigned short) * lidx.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess) e = hipMemcpy(t.d_lidx_node, lidx_node.data(), sizeof(unsigned short) * lidx_node.size(), hipMemcpyHostToDevice);
     if (!rc && e != hipSuccess)
         rc = fail(SWE_HIP_EDEVICE, std::string("tile upload: ") + hipGetErrorString(e));
     if (rc)
@@ -492,6 +498,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "static data is NULL");
     if (o->members < 1 || o->members > 4096)
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
+    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_QUADW)
+        return fail(SWE_HIP_EINVAL, "kernel must be one of SWE_HIP_KERNEL_*");
     if (o->prefetch != 0 && o->prefetch != 4 && o->prefetch != 8)
         return fail(SWE_HIP_EINVAL, "prefetch must be 0, 4 or 8");
     if (o->node_order != SWE_HIP_ORDER_HOST && o->node_order != SWE_HIP_ORDER_MORTON)
@@ -541,8 +549,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->d_geo = m->d_geo;
     c->B = o->members;
     c->MB = mb;
-    c->kernel = (o->kernel == SWE_HIP_KERNEL_LANE || o->kernel == SWE_HIP_KERNEL_QUADW)
-                    ? o->kernel : SWE_HIP_KERNEL_QUAD;
+    c->kernel = (o->kernel == SWE_HIP_KERNEL_LANE || o->kernel == SWE_HIP_KERNEL_QUAD)
+                    ? o->kernel : SWE_HIP_KERNEL_QUADW;
     c->prefetch = o->prefetch;
     c->flags = o->flags;
     int rc = 0;
@@ -565,11 +573,18 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         rc = check_launch("k_set_int");
     }
     if (!rc && o->tiled) {
-        if (c->B < 2 || (c->MB != 4 && c->MB != 8 && c->MB != 16))
+        if (c->MB == 32 && c->B >= 32) {
+            if (c->nnbr != 31)
+                rc = fail(SWE_HIP_EINVAL, "the member-lane tiled kernel (member block 32) needs "
+                                          "a 31-point stencil");
+            else
+                rc = build_tiles(c, kTmemTile, kTmemRows);
+        } else if (c->B < 2 || (c->MB != 4 && c->MB != 8 && c->MB != 16)) {
             rc = fail(SWE_HIP_EINVAL, "the tiled ensemble kernel needs members >= 2 and a "
-                                      "member block of 4, 8 or 16");
-        else
+                                      "member block of 4, 8, 16 or 32");
+        } else {
             rc = build_tiles(c, kTiledBlock / c->MB, kTiledLds / (32 * c->MB));
+        }
     }
     if (!rc)
         check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");

@@ -25,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 namespace swe {
 
 constexpr int GEO_N = 17;
@@ -196,6 +198,49 @@ __device__ __forceinline__ double quad_bcast(double v)
     return __hiloint2double(hi, lo);
 }
 
+template <int K>
+__device__ __forceinline__ int quad_bcast_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, K * 0x55, 0xF, 0xF, false);
+}
+
+// f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N-1>),
+// in order: a loop whose index is a compile-time constant in the body.
+template <typename F, int... J>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, J...>)
+{
+    (f(std::integral_constant<int, J>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Geometry of a node, read once per quad: lane k loads fields k, k+4, ...
+// and the quad shares them by DPP (17 loads per node instead of 4 x 17).
+__device__ __forceinline__ Geo load_geo_quad(const double *__restrict__ geo, int ld, int i, int k)
+{
+    double gv[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const int f = 4 * r + k;
+        gv[r] = geo[(long)(f < GEO_N ? f : 0) * ld + i];
+    }
+    double gf[GEO_N];
+    static_for<GEO_N>([&](auto fc) {
+        constexpr int f = decltype(fc)::value;
+        gf[f] = quad_bcast<f & 3>(gv[f >> 2]);
+    });
+    Geo g;
+    g.x = gf[G_X]; g.y = gf[G_Y]; g.z = gf[G_Z]; g.f = gf[G_F]; g.ghm = gf[G_GHM];
+    g.pu0 = gf[G_PU]; g.pu1 = gf[G_PU + 1]; g.pu2 = gf[G_PU + 2];
+    g.pv0 = gf[G_PV]; g.pv1 = gf[G_PV + 1]; g.pv2 = gf[G_PV + 2];
+    g.pw0 = gf[G_PW]; g.pw1 = gf[G_PW + 1]; g.pw2 = gf[G_PW + 2];
+    g.gx = gf[G_GG]; g.gy = gf[G_GG + 1]; g.gz = gf[G_GG + 2];
+    return g;
+}
+
 // Stencil-slot loop shared by the stage kernels.  PF == 0 leaves the load
 // schedule to the compiler.  PF = D > 0 software-pipelines it: all neighbour
 // ids are loaded first, then a ring of D slots is kept in flight -- each
@@ -303,11 +348,14 @@ struct QuadWSlot {
     double s;
 };
 
-// Single state, quad mapping with shared weight loads: lane k of the quad
-// loads only weight k (Dx, Dy, Dz or L) of each slot -- one 8-byte load per
-// lane, a wave's 16 nodes x 32 B read once instead of four times -- and the
-// quad exchanges the four weights by DPP.  Same sums, same order as
-// k_stage_quad.
+// Single state, quad mapping with shared loads: lane k of the quad loads
+// only weight k (Dx, Dy, Dz or L) of each slot -- one 8-byte load per lane,
+// a wave's 16 nodes x 32 B read once instead of four times -- and the quad
+// exchanges the four weights by DPP; the stencil ids, the node's own state
+// and its geometry are likewise read once per quad and shared by DPP.  A
+// wave64 memory instruction costs the CU's address unit the same time
+// whatever it fetches, so every load removed counts.  Same sums, same order
+// as k_stage_quad.
 template <int STAGE, bool EXACT, int NNBR, int PF>
 __global__ __launch_bounds__(256) void k_stage_quadw(DevMesh m, DevStage st)
 {
@@ -338,7 +386,23 @@ __global__ __launch_bounds__(256) void k_stage_quadw(DevMesh m, DevStage st)
         sz = madd<EXACT>(sz, wz, r.s);
         sl = madd<EXACT>(sl, wl, r.s);
     };
-    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (QuadWSlot *)nullptr);
+    if constexpr (NNBR > 0) {
+        // Stencil ids once per quad too: lane k loads slots k, k+4, ...;
+        // slot j's id comes from lane j & 3 by DPP.
+        constexpr int NQ = (NNBR + 3) / 4;
+        int qv[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int j = 4 * i + k;
+            qv[i] = nb[(j < NNBR ? j : 0) * ld];
+        }
+        static_for<NNBR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            acc(load(j, quad_bcast_i<j & 3>(qv[j >> 2])));
+        });
+    } else {
+        stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (QuadWSlot *)nullptr);
+    }
     Sums S;
     S.dudx = quad_bcast<0>(sx); S.dudy = quad_bcast<0>(sy);
     S.dudz = quad_bcast<0>(sz); S.lu = quad_bcast<0>(sl);
@@ -349,11 +413,11 @@ __global__ __launch_bounds__(256) void k_stage_quadw(DevMesh m, DevStage st)
     S.dhdx = quad_bcast<3>(sx); S.dhdy = quad_bcast<3>(sy);
     S.dhdz = quad_bcast<3>(sz); S.lh = quad_bcast<3>(sl);
 
-    const double2 s01 = *reinterpret_cast<const double2 *>(src + 4L * node);
-    const double2 s23 = *reinterpret_cast<const double2 *>(src + 4L * node + 2);
-    const Geo g = load_geo(m.geo, m.ld, node);
+    const double own = src[4L * node + k];
+    const Geo g = load_geo_quad(m.geo, m.ld, node, k);
     double F[4];
-    rhs_point(S, s01.x, s01.y, s23.x, s23.y, g, m.gh0, F);
+    rhs_point(S, quad_bcast<0>(own), quad_bcast<1>(own), quad_bcast<2>(own), quad_bcast<3>(own),
+              g, m.gh0, F);
     const double Fk = k == 0 ? F[0] : k == 1 ? F[1] : k == 2 ? F[2] : F[3];
     if (live)
         stage_update<STAGE, EXACT>(st, 4L * node + k, Fk);
@@ -452,9 +516,11 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
         // 2j+1 fetch its (Dx, Dy) and (Dz, L), one vector load each, up
         // front.  The weights go to a wave-private LDS row and are read back
         // per slot with uniform-address (broadcast) LDS loads; the id moves
-        // to a scalar register with v_readlane.  No scalar-memory round trip
-        // sits between the neighbour gathers, so several slots of gathers
-        // stay in flight.
+        // to a scalar register with v_readlane.  PF > 0: the neighbour
+        // gathers run as a rolling pipeline PF slots deep (slot j + PF is
+        // issued right after slot j is consumed); PF == 0 leaves the order
+        // to the compiler, which issues them in batches of 3-4 slots.
+        constexpr int D = PF > 0 ? PF : 6;
         __shared__ double2 wrow[4][64];
         const int lane = (int)threadIdx.x & 63;
         const int wid = (int)threadIdx.x >> 6;
@@ -464,13 +530,22 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
         wrow[wid][lane] = *reinterpret_cast<const double2 *>(wp + 4L * jw * ld + 2 * (lane & 1));
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double ru[D], rv[D], rw[D], rh[D];
+        auto issue = [&](int j, int q) {
+            const double *sq = src + 4L * q * B;
+            ru[j % D] = sq[0];
+            rv[j % D] = sq[B];
+            rw[j % D] = sq[2 * B];
+            rh[j % D] = sq[3 * B];
+        };
+#pragma unroll
+        for (int j = 0; j < D && j < NNBR; ++j)
+            issue(j, __builtin_amdgcn_readlane(qv, j));
 #pragma unroll
         for (int j = 0; j < NNBR; ++j) {
-            const int q = __builtin_amdgcn_readlane(qv, j);
+            const double su = ru[j % D], sv = rv[j % D], sw = rw[j % D], sh = rh[j % D];
             const double2 w01 = wrow[wid][2 * j];
             const double2 w23 = wrow[wid][2 * j + 1];
-            const double *sq = src + 4L * q * B;
-            const double su = sq[0], sv = sq[B], sw = sq[2 * B], sh = sq[3 * B];
             S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
             S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
             S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
@@ -479,9 +554,20 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
             S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
             S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
             S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
-            if constexpr (PF > 0) {
-                if (j % PF == PF - 1)
-                    __builtin_amdgcn_sched_barrier(0);
+            if (j + D < NNBR) {
+                // Empty asm: ties slot j + D's id to all 16 sums after slot
+                // j, so slot j is fully consumed (its registers free) before
+                // the gathers of slot j + D issue.  (sched_barrier cannot
+                // hold this order: it carries no dependency and is hoisted.)
+                int q = __builtin_amdgcn_readlane(qv, j + D);
+                if constexpr (PF > 0)
+                    asm volatile(""
+                                 : "+s"(q)
+                                 : "v"(S.dudx), "v"(S.dvdx), "v"(S.dwdx), "v"(S.dhdx),
+                                   "v"(S.dudy), "v"(S.dvdy), "v"(S.dwdy), "v"(S.dhdy),
+                                   "v"(S.dudz), "v"(S.dvdz), "v"(S.dwdz), "v"(S.dhdz),
+                                   "v"(S.lu), "v"(S.lv), "v"(S.lw), "v"(S.lh));
+                issue(j + D, q);
             }
         }
     } else {
@@ -524,6 +610,157 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
     }
 }
 
+// Stage update of two adjacent members (elements e, e + 1; e even) with
+// 16-byte loads and stores; per element the arithmetic of stage_update.
+template <int STAGE, bool EXACT>
+__device__ __forceinline__ void stage_update_pair(const DevStage &st, long e, double F0, double F1)
+{
+    double2 *out = reinterpret_cast<double2 *>(st.out + e);
+    if constexpr (STAGE == ST_RHS) {
+        *out = make_double2(F0, F1);
+    } else if constexpr (STAGE == ST_0) {
+        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
+        *reinterpret_cast<double2 *>(st.d + e) = make_double2(F0, F1);
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, F0), axpy<EXACT>(hv.y, st.c, F1));
+    } else if constexpr (STAGE == ST_1 || STAGE == ST_2) {
+        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
+        double2 *dp = reinterpret_cast<double2 *>(st.d + e);
+        const double2 d0 = *dp;
+        *dp = make_double2(axpy<EXACT>(d0.x, st.dcoef, F0), axpy<EXACT>(d0.y, st.dcoef, F1));
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, F0), axpy<EXACT>(hv.y, st.c, F1));
+    } else {
+        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
+        const double2 d0 = *reinterpret_cast<const double2 *>(st.d + e);
+        const double dv0 = axpy<EXACT>(d0.x, st.dcoef, F0);
+        const double dv1 = axpy<EXACT>(d0.y, st.dcoef, F1);
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, dv0), axpy<EXACT>(hv.y, st.c, dv1));
+    }
+}
+
+// One stencil slot into the 16 sums (the reference's term order).
+template <bool EXACT>
+__device__ __forceinline__ void acc_slot(Sums &S, double2 w01, double2 w23, double su, double sv,
+                                         double sw, double sh)
+{
+    S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
+    S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
+    S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
+    S.dwdy = madd<EXACT>(S.dwdy, w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, w01.y, sh);
+    S.dudz = madd<EXACT>(S.dudz, w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, w23.x, sv);
+    S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
+    S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
+    S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble, member pairs (member block 64, B a multiple of 64).  A wave64
+// memory instruction costs the CU's texture addresser a fixed ~16 cycles
+// whatever its width, so an 8-byte-per-lane gather (one member per lane)
+// moves only 32 B per addresser cycle.  Here lane (half h, pair p) of a wave
+// holds members 2p, 2p+1 of node 2*wave + h and gathers them with one
+// 16-byte load per variable: 4 loads per slot for 2 nodes x 64 members,
+// half the instructions of one member per lane.  Each half-wave's node
+// weights sit in a wave-private LDS row read back with uniform-per-half
+// broadcasts; its neighbour ids come from one vector load and v_readlane.
+// Per member the sums and the stage update are those of k_stage_members.
+template <int STAGE, bool EXACT, int NNBR, int DEPTH>
+__global__ __launch_bounds__(256) void k_stage_mpair(DevMesh m, DevStage st)
+{
+    static_assert(NNBR > 0 && NNBR <= 32, "one slot per half-wave lane");
+    static_assert(DEPTH >= 1, "pipeline depth");
+    if ((int)blockIdx.y * 64 >= *st.active)
+        return;   // whole member block idle (uniform over the workgroup)
+    const int lane = (int)threadIdx.x & 63;
+    const int wid = (int)threadIdx.x >> 6;
+    const int half = lane >> 5;
+    const int p = lane & 31;
+    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int node_raw = t >> 5;
+    const bool live = node_raw < m.n;
+    const int node = live ? node_raw : m.n - 1;
+    const long ld = slot_stride(m);
+    const long sb = slot_base<NNBR>(m, node);
+    const long B = st.members;
+    const int mem = (int)blockIdx.y * 64 + 2 * p;
+    const double *__restrict__ src = st.src + mem;
+    const int *__restrict__ nb = m.nbr + sb;
+    const double *__restrict__ wp = m.wts + 4L * sb;
+
+    __shared__ double2 wrow[4][2][2 * NNBR];
+    const int jq = p < NNBR ? p : 0;
+    const int qv = nb[jq * ld];
+    {
+        const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * jq * ld);
+        const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * jq * ld + 2);
+        if (p < NNBR) {
+            wrow[wid][half][2 * p] = w01;
+            wrow[wid][half][2 * p + 1] = w23;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    // Rolling pipeline DEPTH slots deep: slot j + DEPTH is issued once slot j
+    // is consumed (the empty asm ties its id to all 32 sums after slot j).
+    auto id_of = [&](int j) {
+        const int qa = __builtin_amdgcn_readlane(qv, j);
+        const int qb = __builtin_amdgcn_readlane(qv, 32 + j);
+        return half ? qb : qa;
+    };
+    double2 ru[DEPTH], rv[DEPTH], rw[DEPTH], rh[DEPTH];
+    auto issue = [&](int j, int q) {
+        const double *sq = src + 4L * q * B;
+        ru[j % DEPTH] = *reinterpret_cast<const double2 *>(sq);
+        rv[j % DEPTH] = *reinterpret_cast<const double2 *>(sq + B);
+        rw[j % DEPTH] = *reinterpret_cast<const double2 *>(sq + 2 * B);
+        rh[j % DEPTH] = *reinterpret_cast<const double2 *>(sq + 3 * B);
+    };
+    Sums S0 = {}, S1 = {};
+#pragma unroll
+    for (int j = 0; j < DEPTH && j < NNBR; ++j)
+        issue(j, id_of(j));
+#pragma unroll
+    for (int j = 0; j < NNBR; ++j) {
+        const double2 su = ru[j % DEPTH], sv = rv[j % DEPTH], sw = rw[j % DEPTH],
+                      sh = rh[j % DEPTH];
+        const double2 w01 = wrow[wid][half][2 * j];
+        const double2 w23 = wrow[wid][half][2 * j + 1];
+        acc_slot<EXACT>(S0, w01, w23, su.x, sv.x, sw.x, sh.x);
+        acc_slot<EXACT>(S1, w01, w23, su.y, sv.y, sw.y, sh.y);
+        if (j + DEPTH < NNBR) {
+            int q = id_of(j + DEPTH);
+            asm volatile(""
+                         : "+v"(q)
+                         : "v"(S0.dudx), "v"(S0.dvdx), "v"(S0.dwdx), "v"(S0.dhdx),
+                           "v"(S0.dudy), "v"(S0.dvdy), "v"(S0.dwdy), "v"(S0.dhdy),
+                           "v"(S0.dudz), "v"(S0.dvdz), "v"(S0.dwdz), "v"(S0.dhdz),
+                           "v"(S0.lu), "v"(S0.lv), "v"(S0.lw), "v"(S0.lh));
+            asm volatile(""
+                         : "+v"(q)
+                         : "v"(S1.dudx), "v"(S1.dvdx), "v"(S1.dwdx), "v"(S1.dhdx),
+                           "v"(S1.dudy), "v"(S1.dvdy), "v"(S1.dwdy), "v"(S1.dhdy),
+                           "v"(S1.dudz), "v"(S1.dvdz), "v"(S1.dwdz), "v"(S1.dhdz),
+                           "v"(S1.lu), "v"(S1.lv), "v"(S1.lw), "v"(S1.lh));
+            issue(j + DEPTH, q);
+        }
+    }
+    const double *own = src + 4L * node * B;
+    const double2 u = *reinterpret_cast<const double2 *>(own);
+    const double2 v = *reinterpret_cast<const double2 *>(own + B);
+    const double2 w = *reinterpret_cast<const double2 *>(own + 2 * B);
+    const double2 h = *reinterpret_cast<const double2 *>(own + 3 * B);
+    const Geo g = load_geo(m.geo, m.ld, node);
+    double F0[4], F1[4];
+    rhs_point(S0, u.x, v.x, w.x, h.x, g, m.gh0, F0);
+    rhs_point(S1, u.y, v.y, w.y, h.y, g, m.gh0, F1);
+    if (live) {
+        const long e = 4L * node * B + mem;
+#pragma unroll
+        for (int var = 0; var < 4; ++var)
+            stage_update_pair<STAGE, EXACT>(st, e + var * B, F0[var], F1[var]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Ensemble, tiled (B >= 2): the mesh is stored in a space-filling-curve node
 // order and cut into tiles of consecutive nodes whose stencils share few
@@ -538,6 +775,7 @@ struct DevTiles {
     const int *ustart;              // [ntiles + 1] offsets into ulist
     const int *ulist;               // distinct stencil nodes of each tile, ascending
     const unsigned short *lidx;     // [nnbr][ld] position of nbr(j, node) in its tile's list
+    const unsigned short *lidx_node;  // [n][32] the same, node-major (64 B per node)
     int ntiles;
     int nmb;                        // member blocks per tile (B / MB)
 };
@@ -609,6 +847,178 @@ __global__ __launch_bounds__(512) void k_stage_tiled(DevMesh m, DevTiles tl, Dev
 #pragma unroll
     for (int var = 0; var < 4; ++var)
         stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
+}
+
+// Wave-uniform loads through the constant address space (read-only, so a
+// scalar load even after stores the compiler cannot rule out as aliases),
+// pinned to scalar registers: the empty asm keeps the compiler from turning
+// "c ? load(b) : load(a)" into one load from a per-lane address.
+__device__ __forceinline__ unsigned s_u32(const unsigned *q)
+{
+    unsigned x = *(const __attribute__((address_space(4))) unsigned *)q;
+    asm volatile("" : "+s"(x));
+    return x;
+}
+__device__ __forceinline__ double s_f64(const double *q)
+{
+    double x = *(const __attribute__((address_space(4))) double *)q;
+    asm volatile("" : "+s"(x));
+    return x;
+}
+
+// Geometry of one of two wave-uniform nodes (a for the lower half wave, b
+// for the upper), both read with scalar loads.
+__device__ __forceinline__ Geo load_geo_pair(const double *__restrict__ geo, int ld, int a, int b,
+                                             bool upper)
+{
+    double gf[GEO_N];
+#pragma unroll
+    for (int f = 0; f < GEO_N; ++f) {
+        const double xa = s_f64(geo + (long)f * ld + a);
+        const double xb = s_f64(geo + (long)f * ld + b);
+        gf[f] = upper ? xb : xa;
+    }
+    Geo g;
+    g.x = gf[G_X]; g.y = gf[G_Y]; g.z = gf[G_Z]; g.f = gf[G_F]; g.ghm = gf[G_GHM];
+    g.pu0 = gf[G_PU]; g.pu1 = gf[G_PU + 1]; g.pu2 = gf[G_PU + 2];
+    g.pv0 = gf[G_PV]; g.pv1 = gf[G_PV + 1]; g.pv2 = gf[G_PV + 2];
+    g.pw0 = gf[G_PW]; g.pw1 = gf[G_PW + 1]; g.pw2 = gf[G_PW + 2];
+    g.gx = gf[G_GG]; g.gy = gf[G_GG + 1]; g.gz = gf[G_GG + 2];
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble, LDS tiles with member lanes (member block 32).  A workgroup
+// takes one tile (<= kTmemTile consecutive Morton nodes whose stencils touch
+// <= kTmemRows distinct nodes) for 32 members.  It copies those neighbour
+// rows into LDS once -- 16-byte global loads, one row (4 variables x 32
+// members, 1 KB) per wave instruction -- so each neighbour row crosses the
+// CU's address unit once per tile instead of once per stencil that names it
+// (~4x fewer bytes than gathering from L2).  Then lane = member, two nodes
+// per wave: per slot the lane reads its member's (u, v, w, h) with two
+// 16-byte LDS loads; stencil positions (16-bit, node-major) and weights are
+// wave-uniform per node and come through scalar loads, one set per half
+// wave.  LDS row layout [row][var pair][member][2 vars] keeps the reads
+// contiguous across lanes.  Per member the sums, the RHS and the stage
+// update are those of k_stage_members (bitwise in exact mode).
+constexpr int kTmemTile = 16;   // nodes per tile (4 waves x 2 nodes x 2 rounds)
+constexpr int kTmemRows = 76;   // distinct rows per tile: 76 KB of LDS, 2 workgroups per CU
+
+template <int STAGE, bool EXACT, int NNBR>
+__global__ __launch_bounds__(256) void k_stage_tmem(DevMesh m, DevTiles tl, DevStage st)
+{
+    static_assert(NNBR > 0 && NNBR <= 32, "node-major positions hold 32 slots");
+    constexpr int MB = 32;
+    extern __shared__ double rows[];   // [ucount][2][32][2]
+    const int lb = xcd_block(st.nblk);
+    const int tile = lb / tl.nmb;
+    const int mb = lb - tile * tl.nmb;
+    if (tile >= tl.ntiles || mb * MB >= *st.active)
+        return;   // uniform over the workgroup
+    const int t0 = tl.tile_start[tile];
+    const int tcount = tl.tile_start[tile + 1] - t0;
+    const int u0 = tl.ustart[tile];
+    const int ucount = tl.ustart[tile + 1] - u0;
+    const long B = st.members;
+    const int col0 = mb * MB;
+    const int lane = (int)threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+
+    // Fill: lane (v, p) moves members 2p, 2p+1 of variable v of a row.
+    {
+        const int v = lane >> 4;
+        const int p = lane & 15;
+        const double *__restrict__ s0 = st.src + col0 + 2 * p + (long)v * B;
+        double *d0 = rows + (v >> 1) * 64 + 4 * p + (v & 1);
+        for (int r = wid; r < ucount; r += 16) {
+            double2 x[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int rr = r + 4 * k;
+                const int u = tl.ulist[u0 + (rr < ucount ? rr : r)];
+                x[k] = *reinterpret_cast<const double2 *>(s0 + 4L * u * B);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int rr = r + 4 * k;
+                if (rr < ucount) {
+                    d0[rr * 128] = x[k].x;
+                    d0[rr * 128 + 2] = x[k].y;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    const int ml = lane & 31;
+    const int half = lane >> 5;
+    const int mem = col0 + ml;
+    const long ld = slot_stride(m);
+    for (int i0 = 2 * wid; i0 < tcount; i0 += 8) {
+        const int nA = t0 + i0;
+        const bool hasB = i0 + 1 < tcount;
+        const int nB = hasB ? nA + 1 : nA;
+        const int node = half ? nB : nA;
+        const bool live = !half || hasB;
+        const unsigned *__restrict__ pA =
+            reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * nA);
+        const unsigned *__restrict__ pB =
+            reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * nB);
+        const double *__restrict__ wA = m.wts + 4L * slot_base<NNBR>(m, nA);
+        const double *__restrict__ wB = m.wts + 4L * slot_base<NNBR>(m, nB);
+        const double *lrow = rows + 2 * ml;
+        auto row_off = [&](int j) {
+            const unsigned a = s_u32(pA + (j >> 1)), b = s_u32(pB + (j >> 1));
+            const int ra = (int)((j & 1) ? (a >> 16) : (a & 0xffffu));
+            const int rb = (int)((j & 1) ? (b >> 16) : (b & 0xffffu));
+            return (half ? rb : ra) * 128;
+        };
+        // LDS reads run DEPTH slots ahead; slot j + DEPTH's address is tied
+        // to the sums after slot j (bounded live ranges in exact mode too).
+        constexpr int DEPTH = 3;
+        double2 ruv[DEPTH], rwh[DEPTH];
+        auto issue = [&](int j, int off) {
+            ruv[j % DEPTH] = *reinterpret_cast<const double2 *>(lrow + off);
+            rwh[j % DEPTH] = *reinterpret_cast<const double2 *>(lrow + off + 64);
+        };
+        Sums S = {};
+        static_for<DEPTH>([&](auto jc) { issue(decltype(jc)::value, row_off(decltype(jc)::value)); });
+        static_for<NNBR>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const double *wa = wA + 4L * j * ld;
+            const double *wb = wB + 4L * j * ld;
+            double wv[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double xa = s_f64(wa + c), xb = s_f64(wb + c);
+                wv[c] = half ? xb : xa;
+            }
+            const double2 uv = ruv[j % DEPTH], wh = rwh[j % DEPTH];
+            acc_slot<EXACT>(S, make_double2(wv[0], wv[1]), make_double2(wv[2], wv[3]), uv.x,
+                            uv.y, wh.x, wh.y);
+            if constexpr (j + DEPTH < NNBR) {
+                int off = row_off(j + DEPTH);
+                asm volatile(""
+                             : "+v"(off)
+                             : "v"(S.dudx), "v"(S.dvdx), "v"(S.dwdx), "v"(S.dhdx),
+                               "v"(S.dudy), "v"(S.dvdy), "v"(S.dwdy), "v"(S.dhdy),
+                               "v"(S.dudz), "v"(S.dvdz), "v"(S.dwdz), "v"(S.dhdz),
+                               "v"(S.lu), "v"(S.lv), "v"(S.lw), "v"(S.lh));
+                issue(j + DEPTH, off);
+            }
+        });
+        const long e = 4L * node * B + mem;
+        const double *__restrict__ src = st.src;
+        const double u = src[e], v = src[e + B], w = src[e + 2 * B], h = src[e + 3 * B];
+        const Geo g = load_geo_pair(m.geo, m.ld, nA, nB, half);
+        double F[4];
+        rhs_point(S, u, v, w, h, g, m.gh0, F);
+        if (live) {
+#pragma unroll
+            for (int var = 0; var < 4; ++var)
+                stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
+        }
+    }
 }
 
 }  // namespace swe

@@ -67,6 +67,24 @@ def test_bad_static_data_rejected(pkg):
     assert L.swe_hip_create(C.byref(s), 1, 0, 0, C.byref(out)) == -1
 
 
+def test_bad_options_rejected_before_device(pkg):
+    """Out-of-range options fail with EINVAL (no device needed to find out)."""
+    path = pkg.mesh.case_path(642, seed=1)
+    prob = pkg.host.Problem(path, use_rcm=True, layout="cfdl")
+    s, keep = pkg.runtime.static_from_problem(prob)
+    L = pkg.runtime.lib()
+    base = dict(members=1, flags=0, device=0, member_block=0, share=None, kernel=0,
+                prefetch=0, node_order=0, tiled=0, slot_block=0)
+    for field, bad in [("kernel", 4), ("kernel", -1), ("prefetch", 2), ("node_order", 2),
+                       ("slot_block", 8), ("members", 0), ("member_block", 3)]:
+        o = pkg.runtime.SweHipOptions(**dict(base, **{field: bad}))
+        if field == "member_block":
+            o.members = 8
+        out = C.c_void_p()
+        assert L.swe_hip_create_ex(C.byref(s), C.byref(o), C.byref(out)) == -1, field
+        assert out.value is None
+
+
 def test_no_device_is_a_loud_error(pkg):
     import torch
     if torch.cuda.is_available():

@@ -104,7 +104,7 @@ def test_set_active_and_move_member(pkg):
         assert np.array_equal(solo.get_state(), got)
 
 
-@pytest.mark.parametrize("B,mb", [(8, 4), (16, 8), (32, 16), (64, 8)])
+@pytest.mark.parametrize("B,mb", [(8, 4), (16, 8), (32, 16), (64, 8), (32, 32), (64, 32)])
 def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb):
     """LDS-staged kernel over a Morton-ordered device copy: bitwise equal to
     the oracle (node order and staging never change a node's arithmetic)."""
@@ -121,11 +121,12 @@ def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb):
             assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 4)), m
 
 
-def test_tiled_fast_composition_and_active(pkg):
+@pytest.mark.parametrize("mb", [16, 32])
+def test_tiled_fast_composition_and_active(pkg, mb):
     prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl")
-    print("case: tiled fast B=64 N=10242 vs solo", flush=True)
+    print(f"case: tiled fast B=64 mb={mb} N=10242 vs solo", flush=True)
     H0 = _members(pkg, prob, 64)
-    with pkg.runtime.HipContext(prob, members=64, member_block=16, order="morton",
+    with pkg.runtime.HipContext(prob, members=64, member_block=mb, order="morton",
                                 tiled=True) as ctx:
         for m in range(64):
             ctx.set_state(H0[m], m)
@@ -158,7 +159,7 @@ def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk):
         assert np.array_equal(ctx.get_state(), want)
 
 
-@pytest.mark.parametrize("tiled,mb", [(False, 64), (False, 8), (True, 8)])
+@pytest.mark.parametrize("tiled,mb", [(False, 64), (False, 8), (True, 8), (True, 32)])
 def test_blocked_stencil_layout_members_exact(pkg, pyoracle, tiled, mb):
     """Node-blocked stencil arrays (slot_block=16, N=642 is not a multiple
     of 16) give the oracle's bits for the ensemble kernels too."""
