@@ -151,9 +151,10 @@ def ensemble_leg(pkg, torch, steps=500):
     single.set_state(inits[0])
     ms1 = timed(single, steps)
     base = pkg.runtime.HipContext(prob, order="morton")
-    candidates = [("member pairs, pipeline depth 3", dict(member_block=64, prefetch=0)),
-                  ("member pairs, pipeline depth 4", dict(member_block=64, prefetch=8)),
-                  ("LDS tiles, member block 8", dict(member_block=8, tiled=True))]
+    candidates = [("member pairs (16-byte gathers), pipeline depth 3",
+                   dict(member_block=64, prefetch=0)),
+                  ("LDS tiles with member lanes, member block 32, persistent",
+                   dict(member_block=32, tiled=True))]
     best = None
     for name, kw in candidates:
         ctx = pkg.runtime.HipContext(prob, members=64, share=base, **kw)

@@ -12,6 +12,7 @@ DevTiles tiles_of(const swe_hip_ctx *c, int mb)
     tl.ulist = c->tiles.d_ulist;
     tl.lidx = c->tiles.d_lidx;
     tl.lidx_node = c->tiles.d_lidx_node;
+    tl.ulist_tile = c->tiles.d_ulist_tile;
     tl.ntiles = c->tiles.ntiles;
     tl.nmb = c->B / mb;
     return tl;
@@ -31,8 +32,13 @@ template <int STAGE, bool EX>
 void launch_tmem(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
     const DevTiles tl = tiles_of(c, 32);
-    st.nblk = (int)(((long)tl.ntiles * tl.nmb + 7) / 8 * 8);
-    const size_t lds = sizeof(double) * 4 * 32 * (size_t)c->tiles.umax;
+    // persistent: two workgroups per CU (LDS-bound), each over a contiguous
+    // chunk of (tile, member block) items
+    const long items = (long)tl.ntiles * tl.nmb;
+    const long wgs = 2L * c->ncu;
+    st.nblk = (int)(((items < wgs ? items : wgs) + 7) / 8 * 8);
+    const size_t lds =
+        sizeof(double) * (kTmemWtsDoubles + kTmemGeoDoubles + 4 * 32 * (size_t)c->tiles.umax);
     k_stage_tmem<STAGE, EX, 31><<<dim3(st.nblk), dim3(256), lds, c->stream>>>(m, tl, st);
 }
 

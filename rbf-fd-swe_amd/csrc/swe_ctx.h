@@ -58,6 +58,7 @@ struct SweTiles {
     int *d_tile_start = nullptr, *d_ustart = nullptr, *d_ulist = nullptr;
     unsigned short *d_lidx = nullptr;
     unsigned short *d_lidx_node = nullptr;   // [n][32] node-major copy of d_lidx
+    int *d_ulist_tile = nullptr;             // [ntiles][kTmemRows] (member-lane kernel)
 };
 
 struct swe_hip_ctx {
@@ -74,6 +75,7 @@ struct swe_hip_ctx {
     int B = 1, MB = 1;
     int kernel = SWE_HIP_KERNEL_QUAD;   // single-state mapping
     int prefetch = 0;                   // stencil slots per load batch (0, 4, 8)
+    int ncu = 0;                        // compute units of the device
     unsigned flags = 0;
     double *d_H = nullptr, *d_Ka = nullptr, *d_Kb = nullptr, *d_D = nullptr, *d_F = nullptr;
     double *d_stage = nullptr;   // [n*4] staging for member upload/read-back

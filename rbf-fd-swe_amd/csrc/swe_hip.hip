@@ -219,7 +219,7 @@ void free_ctx(swe_hip_ctx *c)
     if (c->graphN) (void)hipGraphExecDestroy(c->graphN);
     void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
-                    c->tiles.d_lidx, c->tiles.d_lidx_node};
+                    c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -446,6 +446,14 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
         ustart.push_back((int)ulist.size());
         ++tile;
     }
+    // fixed-stride copy for the member-lane kernel (tiles of <= kTmemRows rows)
+    std::vector<int> ulist_tile;
+    if (umax <= kTmemRows) {
+        ulist_tile.assign((size_t)tile * kTmemRows, 0);
+        for (int k = 0; k < tile; ++k)
+            std::copy(ulist.begin() + ustart[k], ulist.begin() + ustart[k + 1],
+                      ulist_tile.begin() + (size_t)k * kTmemRows);
+    }
     SweTiles &t = c->tiles;
     t.ntiles = tile;
     t.tmax = tmax;
@@ -456,12 +464,16 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_ulist, sizeof(int) * ulist.size());
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_lidx, sizeof(unsigned short) * lidx.size());
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_lidx_node, sizeof(unsigned short) * lidx_node.size());
+    if (!rc && !ulist_tile.empty())
+        rc = dev_alloc(&c->bytes, (void **)&t.d_ulist_tile, sizeof(int) * ulist_tile.size());
     hipError_t e = hipSuccess;
     if (!rc) e = hipMemcpy(t.d_tile_start, tstart.data(), sizeof(int) * tstart.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) e = hipMemcpy(t.d_ustart, ustart.data(), sizeof(int) * ustart.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) e = hipMemcpy(t.d_ulist, ulist.data(), sizeof(int) * ulist.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) e = hipMemcpy(t.d_lidx, lidx.data(), sizeof(unsigned short) * lidx.size(), hipMemcpyHostToDevice);
     if (!rc && e == hipSuccess) e = hipMemcpy(t.d_lidx_node, lidx_node.data(), sizeof(unsigned short) * lidx_node.size(), hipMemcpyHostToDevice);
+    if (!rc && e == hipSuccess && !ulist_tile.empty())
+        e = hipMemcpy(t.d_ulist_tile, ulist_tile.data(), sizeof(int) * ulist_tile.size(), hipMemcpyHostToDevice);
     if (!rc && e != hipSuccess)
         rc = fail(SWE_HIP_EDEVICE, std::string("tile upload: ") + hipGetErrorString(e));
     if (rc)
@@ -553,6 +565,9 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
                     ? o->kernel : SWE_HIP_KERNEL_QUADW;
     c->prefetch = o->prefetch;
     c->flags = o->flags;
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess || c->ncu <= 0)
+        c->ncu = 256;
     int rc = 0;
     auto check = [&](hipError_t e, const char *what) {
         if (e != hipSuccess && !rc)

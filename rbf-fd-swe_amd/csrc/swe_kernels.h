@@ -774,6 +774,8 @@ struct DevTiles {
     const int *tile_start;          // [ntiles + 1] first node of each tile
     const int *ustart;              // [ntiles + 1] offsets into ulist
     const int *ulist;               // distinct stencil nodes of each tile, ascending
+    const int *ulist_tile;          // the same at a fixed stride: [ntiles][kTmemRows],
+                                    // zero-padded (member-lane kernel only)
     const unsigned short *lidx;     // [nnbr][ld] position of nbr(j, node) in its tile's list
     const unsigned short *lidx_node;  // [n][32] the same, node-major (64 B per node)
     int ntiles;
@@ -851,19 +853,25 @@ __global__ __launch_bounds__(512) void k_stage_tiled(DevMesh m, DevTiles tl, Dev
 
 // Wave-uniform loads through the constant address space (read-only, so a
 // scalar load even after stores the compiler cannot rule out as aliases),
-// pinned to scalar registers: the empty asm keeps the compiler from turning
-// "c ? load(b) : load(a)" into one load from a per-lane address.
+// pinned to scalar registers: the empty (non-volatile, freely scheduled) asm
+// keeps the compiler from turning "c ? load(b) : load(a)" into one load from
+// a per-lane address.
 __device__ __forceinline__ unsigned s_u32(const unsigned *q)
 {
     unsigned x = *(const __attribute__((address_space(4))) unsigned *)q;
-    asm volatile("" : "+s"(x));
+    asm("" : "+s"(x));
     return x;
 }
 __device__ __forceinline__ double s_f64(const double *q)
 {
     double x = *(const __attribute__((address_space(4))) double *)q;
-    asm volatile("" : "+s"(x));
+    asm("" : "+s"(x));
     return x;
+}
+// Wave-uniform load with no pin (nothing selects between two of them).
+__device__ __forceinline__ int c_i32(const int *q)
+{
+    return *(const __attribute__((address_space(4))) int *)q;
 }
 
 // Geometry of one of two wave-uniform nodes (a for the lower half wave, b
@@ -890,112 +898,191 @@ __device__ __forceinline__ Geo load_geo_pair(const double *__restrict__ geo, int
 // ---------------------------------------------------------------------------
 // Ensemble, LDS tiles with member lanes (member block 32).  A workgroup
 // takes one tile (<= kTmemTile consecutive Morton nodes whose stencils touch
-// <= kTmemRows distinct nodes) for 32 members.  It copies those neighbour
-// rows into LDS once -- 16-byte global loads, one row (4 variables x 32
-// members, 1 KB) per wave instruction -- so each neighbour row crosses the
-// CU's address unit once per tile instead of once per stencil that names it
-// (~4x fewer bytes than gathering from L2).  Then lane = member, two nodes
-// per wave: per slot the lane reads its member's (u, v, w, h) with two
-// 16-byte LDS loads; stencil positions (16-bit, node-major) and weights are
-// wave-uniform per node and come through scalar loads, one set per half
-// wave.  LDS row layout [row][var pair][member][2 vars] keeps the reads
-// contiguous across lanes.  Per member the sums, the RHS and the stage
-// update are those of k_stage_members (bitwise in exact mode).
-constexpr int kTmemTile = 16;   // nodes per tile (4 waves x 2 nodes x 2 rounds)
-constexpr int kTmemRows = 76;   // distinct rows per tile: 76 KB of LDS, 2 workgroups per CU
+// <= kTmemRows distinct nodes) for 32 members.  A wave64 global load costs
+// the CU's address unit 16 cycles whatever its width, an LDS b128 read ~5,
+// so the kernel moves every byte it can through LDS:
+//  * the tile's neighbour rows are copied into LDS once -- 16-byte global
+//    loads, one row (4 variables x 32 members, 1 KB) per wave instruction --
+//    instead of being gathered from L2 once per stencil that names them;
+//  * each wave stages the weights (31 x 32 B) and geometry of its two nodes
+//    in a wave-private LDS slice;
+//  * lane = member, two nodes per wave: per slot the lane reads its member's
+//    (u, v, w, h) (two b128 reads) and its half-wave's node weights (two b128
+//    broadcasts); stencil positions (16-bit, node-major, 64 B per node) come
+//    through one scalar load per node.
+// All global loads of an item (rows, weights, geometry, the epilogue's own
+// state, H and D) are issued in one burst, one item ahead of its compute.  LDS row layout [row][var pair][member][2 vars]
+// keeps reads contiguous across lanes.  Per member the sums, the RHS and
+// the stage update are those of k_stage_members (bitwise in exact mode).
+constexpr int kTmemTile = 8;    // nodes per tile: 4 waves x 2 nodes
+constexpr int kTmemRows = 64;   // distinct rows per tile: 64 KB of LDS
+constexpr int kTmemWtsDoubles = 4 * 2 * 32 * 4;   // weights [wave][node][slot][4]
+constexpr int kTmemGeoDoubles = 4 * 2 * 32;       // geometry [wave][node][32]
+constexpr int kTmemFill = kTmemRows / 4;          // rows per wave
 
+template <int STAGE, bool EXACT>
+__device__ __forceinline__ void stage_update_pre(const DevStage &st, long e, double Fv,
+                                                 double hv, double dv)
+{
+    if constexpr (STAGE == ST_RHS) {
+        st.out[e] = Fv;
+    } else if constexpr (STAGE == ST_0) {
+        st.d[e] = Fv;
+        st.out[e] = axpy<EXACT>(hv, st.c, Fv);
+    } else if constexpr (STAGE == ST_1 || STAGE == ST_2) {
+        st.d[e] = axpy<EXACT>(dv, st.dcoef, Fv);
+        st.out[e] = axpy<EXACT>(hv, st.c, Fv);
+    } else {
+        st.out[e] = axpy<EXACT>(hv, st.c, axpy<EXACT>(dv, st.dcoef, Fv));
+    }
+}
+
+// Global loads of one (tile, member block) item, held in registers until
+// they are written to LDS (fill rows, weights, geometry) or used by the
+// epilogue (own state, H, D).
+struct TmemLoads {
+    double2 fx[kTmemFill];
+    double2 xa, xb;
+    double gx;
+    double ou, ov, ow, oh;
+    double hv[4], dv[4];
+};
+
+// Persistent over (tile, member block) items (member blocks of a tile
+// adjacent, tiles in Morton order).  The global loads of a workgroup's next
+// item are issued before its current item is computed from LDS, so their
+// latency hides behind the compute instead of stalling the CU.  No workgroup
+// waits for another: each runs its share of items and exits.
 template <int STAGE, bool EXACT, int NNBR>
 __global__ __launch_bounds__(256) void k_stage_tmem(DevMesh m, DevTiles tl, DevStage st)
 {
     static_assert(NNBR > 0 && NNBR <= 32, "node-major positions hold 32 slots");
     constexpr int MB = 32;
-    extern __shared__ double rows[];   // [ucount][2][32][2]
-    const int lb = xcd_block(st.nblk);
-    const int tile = lb / tl.nmb;
-    const int mb = lb - tile * tl.nmb;
-    if (tile >= tl.ntiles || mb * MB >= *st.active)
-        return;   // uniform over the workgroup
-    const int t0 = tl.tile_start[tile];
-    const int tcount = tl.tile_start[tile + 1] - t0;
-    const int u0 = tl.ustart[tile];
-    const int ucount = tl.ustart[tile + 1] - u0;
+    // weights [4][2][32][4], geometry [4][2][32], rows [ucount][2][32][2]
+    extern __shared__ double lds[];
+    double *wl = lds;
+    double *gl = lds + kTmemWtsDoubles;
+    double *rows = gl + kTmemGeoDoubles;
+    // XCD x (hardware blocks are dealt round-robin over the 8 XCDs) owns
+    // items [x R, (x+1) R); its W workgroups interleave over them (j, j + W,
+    // ...), so the items in flight on one XCD are adjacent tiles that share
+    // halo rows and weights in that XCD's L2.
+    const long items = (long)tl.ntiles * tl.nmb;
+    const long R = (items + 7) / 8;
+    const int xcd = (int)blockIdx.x & 7;
+    const long W = st.nblk >> 3;
+    const long it0 = (long)xcd * R + ((long)blockIdx.x >> 3);
+    const long xend = (long)(xcd + 1) * R;
+    const long it1 = xend < items ? xend : items;
     const long B = st.members;
-    const int col0 = mb * MB;
     const int lane = (int)threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-
-    // Fill: lane (v, p) moves members 2p, 2p+1 of variable v of a row.
-    {
-        const int v = lane >> 4;
-        const int p = lane & 15;
-        const double *__restrict__ s0 = st.src + col0 + 2 * p + (long)v * B;
-        double *d0 = rows + (v >> 1) * 64 + 4 * p + (v & 1);
-        for (int r = wid; r < ucount; r += 16) {
-            double2 x[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int rr = r + 4 * k;
-                const int u = tl.ulist[u0 + (rr < ucount ? rr : r)];
-                x[k] = *reinterpret_cast<const double2 *>(s0 + 4L * u * B);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int rr = r + 4 * k;
-                if (rr < ucount) {
-                    d0[rr * 128] = x[k].x;
-                    d0[rr * 128 + 2] = x[k].y;
-                }
-            }
-        }
-    }
-    __syncthreads();
-
+    const long ld = slot_stride(m);
+    const double *__restrict__ src = st.src;
+    const int active_members = c_i32(st.active);
     const int ml = lane & 31;
     const int half = lane >> 5;
-    const int mem = col0 + ml;
-    const long ld = slot_stride(m);
-    for (int i0 = 2 * wid; i0 < tcount; i0 += 8) {
-        const int nA = t0 + i0;
-        const bool hasB = i0 + 1 < tcount;
-        const int nB = hasB ? nA + 1 : nA;
-        const int node = half ? nB : nA;
-        const bool live = !half || hasB;
-        const unsigned *__restrict__ pA =
-            reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * nA);
-        const unsigned *__restrict__ pB =
-            reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * nB);
-        const double *__restrict__ wA = m.wts + 4L * slot_base<NNBR>(m, nA);
-        const double *__restrict__ wB = m.wts + 4L * slot_base<NNBR>(m, nB);
-        const double *lrow = rows + 2 * ml;
+    const int fv = lane >> 4;
+    const int fp = lane & 15;
+    const int jw = (lane >> 1) < NNBR ? (lane >> 1) : 0;
+    const int gf = ml < GEO_N ? ml : 0;
+
+    // Item geometry (wave-uniform).
+    struct Item {
+        int tile, t0, tcount, ucount, col0, nA, nB;
+        bool live, hasB;
+    };
+    auto item_of = [&](long it) {
+        Item I;
+        const int tile = (int)(it / tl.nmb);
+        const int mb = (int)(it - (long)tile * tl.nmb);
+        I.tile = tile;
+        I.t0 = c_i32(tl.tile_start + tile);
+        I.tcount = c_i32(tl.tile_start + tile + 1) - I.t0;
+        I.ucount = c_i32(tl.ustart + tile + 1) - c_i32(tl.ustart + tile);
+        I.col0 = mb * MB;
+        const int i0 = 2 * wid;
+        I.live = i0 < I.tcount && I.col0 < active_members;
+        I.hasB = i0 + 1 < I.tcount;
+        I.nA = I.t0 + (i0 < I.tcount ? i0 : 0);
+        I.nB = I.hasB ? I.nA + 1 : I.nA;
+        return I;
+    };
+    auto load = [&](const Item &I) {
+        TmemLoads L;
+        L.xa = *reinterpret_cast<const double2 *>(
+            m.wts + 4L * slot_base<NNBR>(m, I.nA) + 4L * jw * ld + 2 * (lane & 1));
+        L.xb = *reinterpret_cast<const double2 *>(
+            m.wts + 4L * slot_base<NNBR>(m, I.nB) + 4L * jw * ld + 2 * (lane & 1));
+        const int node = half ? I.nB : I.nA;
+        L.gx = m.geo[(long)gf * m.ld + node];
+        const double *__restrict__ s0 = src + I.col0 + 2 * fp + (long)fv * B;
+        // wave w copies rows [w * kTmemFill, (w + 1) * kTmemFill) of the
+        // tile: one merged scalar load of ids at a fixed per-tile stride (no
+        // wait for the tile's offset); the zero padding past the tile's
+        // count is a readable node id, loaded but not stored
+        const int *ul = tl.ulist_tile + (long)I.tile * kTmemRows + wid * kTmemFill;
+#pragma unroll
+        for (int k = 0; k < kTmemFill; ++k)
+            L.fx[k] = *reinterpret_cast<const double2 *>(s0 + 4L * c_i32(ul + k) * B);
+        const long e = 4L * node * B + I.col0 + ml;
+        L.ou = src[e]; L.ov = src[e + B]; L.ow = src[e + 2 * B]; L.oh = src[e + 3 * B];
+#pragma unroll
+        for (int var = 0; var < 4; ++var) {
+            L.hv[var] = STAGE != ST_RHS ? st.h[e + var * B] : 0.0;
+            L.dv[var] = (STAGE != ST_RHS && STAGE != ST_0) ? st.d[e + var * B] : 0.0;
+        }
+        return L;
+    };
+    auto to_lds = [&](const Item &I, const TmemLoads &L) {
+        double2 *dst = reinterpret_cast<double2 *>(wl) + wid * 128 + lane;
+        dst[0] = L.xa;    // [wid][0][slot][..]
+        dst[64] = L.xb;   // [wid][1][slot][..]
+        gl[wid * 64 + half * 32 + ml] = L.gx;
+        double *d0 = rows + (fv >> 1) * 64 + 4 * fp + (fv & 1);
+#pragma unroll
+        for (int k = 0; k < kTmemFill; ++k) {
+            const int rr = wid * kTmemFill + k;
+            if (rr < I.ucount) {
+                d0[rr * 128] = L.fx[k].x;
+                d0[rr * 128 + 2] = L.fx[k].y;
+            }
+        }
+    };
+    auto compute = [&](const Item &I, const TmemLoads &L) {
+        unsigned pa[16], pb[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            pa[k] = s_u32(reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * I.nA) + k);
+            pb[k] = s_u32(reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * I.nB) + k);
+        }
         auto row_off = [&](int j) {
-            const unsigned a = s_u32(pA + (j >> 1)), b = s_u32(pB + (j >> 1));
+            const unsigned a = pa[j >> 1], b = pb[j >> 1];
             const int ra = (int)((j & 1) ? (a >> 16) : (a & 0xffffu));
             const int rb = (int)((j & 1) ? (b >> 16) : (b & 0xffffu));
             return (half ? rb : ra) * 128;
         };
-        // LDS reads run DEPTH slots ahead; slot j + DEPTH's address is tied
-        // to the sums after slot j (bounded live ranges in exact mode too).
+        const double *lrow = rows + 2 * ml;
+        const double2 *wrow = reinterpret_cast<const double2 *>(wl) + wid * 128 + half * 64;
+        // All four LDS reads of a slot (member row, weights) run DEPTH slots
+        // ahead, in issue order, so waiting for slot j leaves the later
+        // slots in flight; slot j + DEPTH's reads are tied to the sums after
+        // slot j (bounded live ranges in exact mode too).
         constexpr int DEPTH = 3;
-        double2 ruv[DEPTH], rwh[DEPTH];
+        double2 ruv[DEPTH], rwh[DEPTH], rw01[DEPTH], rw23[DEPTH];
         auto issue = [&](int j, int off) {
             ruv[j % DEPTH] = *reinterpret_cast<const double2 *>(lrow + off);
             rwh[j % DEPTH] = *reinterpret_cast<const double2 *>(lrow + off + 64);
+            rw01[j % DEPTH] = wrow[2 * j];
+            rw23[j % DEPTH] = wrow[2 * j + 1];
         };
         Sums S = {};
-        static_for<DEPTH>([&](auto jc) { issue(decltype(jc)::value, row_off(decltype(jc)::value)); });
+        static_for<DEPTH>(
+            [&](auto jc) { issue(decltype(jc)::value, row_off(decltype(jc)::value)); });
         static_for<NNBR>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
-            const double *wa = wA + 4L * j * ld;
-            const double *wb = wB + 4L * j * ld;
-            double wv[4];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const double xa = s_f64(wa + c), xb = s_f64(wb + c);
-                wv[c] = half ? xb : xa;
-            }
             const double2 uv = ruv[j % DEPTH], wh = rwh[j % DEPTH];
-            acc_slot<EXACT>(S, make_double2(wv[0], wv[1]), make_double2(wv[2], wv[3]), uv.x,
-                            uv.y, wh.x, wh.y);
+            acc_slot<EXACT>(S, rw01[j % DEPTH], rw23[j % DEPTH], uv.x, uv.y, wh.x, wh.y);
             if constexpr (j + DEPTH < NNBR) {
                 int off = row_off(j + DEPTH);
                 asm volatile(""
@@ -1007,17 +1094,47 @@ __global__ __launch_bounds__(256) void k_stage_tmem(DevMesh m, DevTiles tl, DevS
                 issue(j + DEPTH, off);
             }
         });
-        const long e = 4L * node * B + mem;
-        const double *__restrict__ src = st.src;
-        const double u = src[e], v = src[e + B], w = src[e + 2 * B], h = src[e + 3 * B];
-        const Geo g = load_geo_pair(m.geo, m.ld, nA, nB, half);
+        const double *gp = gl + wid * 64 + half * 32;
+        Geo g;
+        g.x = gp[G_X]; g.y = gp[G_Y]; g.z = gp[G_Z]; g.f = gp[G_F]; g.ghm = gp[G_GHM];
+        g.pu0 = gp[G_PU]; g.pu1 = gp[G_PU + 1]; g.pu2 = gp[G_PU + 2];
+        g.pv0 = gp[G_PV]; g.pv1 = gp[G_PV + 1]; g.pv2 = gp[G_PV + 2];
+        g.pw0 = gp[G_PW]; g.pw1 = gp[G_PW + 1]; g.pw2 = gp[G_PW + 2];
+        g.gx = gp[G_GG]; g.gy = gp[G_GG + 1]; g.gz = gp[G_GG + 2];
         double F[4];
-        rhs_point(S, u, v, w, h, g, m.gh0, F);
-        if (live) {
+        rhs_point(S, L.ou, L.ov, L.ow, L.oh, g, m.gh0, F);
+        if (!half || I.hasB) {
+            const int node = half ? I.nB : I.nA;
+            const long e = 4L * node * B + I.col0 + ml;
 #pragma unroll
             for (int var = 0; var < 4; ++var)
-                stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
+                stage_update_pre<STAGE, EXACT>(st, e + var * B, F[var], L.hv[var], L.dv[var]);
         }
+    };
+
+    if (it0 >= it1)
+        return;   // uniform over the workgroup
+    Item cur = item_of(it0);
+    TmemLoads lc = load(cur);
+    to_lds(cur, lc);
+    __syncthreads();
+    for (long it = it0; it < it1; it += W) {
+        const bool more = it + W < it1;
+        Item nxt = cur;
+        TmemLoads ln = lc;
+        if (more) {
+            nxt = item_of(it + W);
+            ln = load(nxt);
+        }
+        if (cur.live)
+            compute(cur, lc);
+        __syncthreads();   // every wave is done reading this item's LDS
+        if (!more)
+            break;
+        to_lds(nxt, ln);
+        __syncthreads();
+        cur = nxt;
+        lc = ln;
     }
 }
 
