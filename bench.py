@@ -63,6 +63,8 @@ def parse_args():
     ap.add_argument("--layout", default="cfdl")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 4 device copies of 1 GiB (HBM counter calibration) and exit")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble leg (N=163842, 64 members, 500 steps)")
     ap.add_argument("--serve", action="store_true",
@@ -208,7 +210,7 @@ def serve_leg(pkg, n_requests, rate, seed=2026):
             print(f"bench: serve {done[0]}/{len(reqs)} done", file=sys.stderr, flush=True)
     work = sum(len(r.inits) * r.nsteps for r in reqs)
     node_rhs = sum(len(r.inits) * r.nsteps * 4 * probs[r.mesh].n_nodes for r in reqs)
-    with pkg.serve.Server(hbm_cap_bytes=200 << 30, slots=64, member_block=16,
+    with pkg.serve.Server(hbm_cap_bytes=200 << 30, slots=64, member_block=64,
                           chunk_steps=16) as srv:
         for p in probs:
             srv.add_mesh(p)
@@ -239,6 +241,16 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        # a plain copy of known size: each dispatch reads and writes 1 GiB
+        a = torch.ones(2**27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate": "copy", "dispatches": 4, "bytes_read_per_dispatch": 2**30,
+                          "bytes_written_per_dispatch": 2**30}), flush=True)
+        return
 
     t0 = time.time()
     path = pkg.mesh.case_path(args.nodes, seed=1)

@@ -1,0 +1,93 @@
+// unit_rates: per-CU throughput of the memory paths the stage kernels choose
+// between on gfx950 -- LDS b128 reads (distinct and broadcast addresses),
+// ds_bpermute, and L1-resident global loads of 16 and 8 bytes per lane.
+// Prints bytes per clock per CU and clocks per wave64 instruction per CU.
+// All indices stay inside their buffers: LDS offsets are masked to 32 KB,
+// global offsets to 16 KB per workgroup.
+// Build: make -C rbf-fd-swe_amd probes; run: rbf-fd-swe_amd/probes/unit_rates
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+
+#define CHECK(x) do { hipError_t e = (x); if (e != hipSuccess) { printf("HIP error %s at %d\n", hipGetErrorString(e), __LINE__); exit(1);} } while (0)
+
+constexpr int ITERS = 4096;
+
+__global__ __launch_bounds__(256) void k_lds(double *out, int bcast)
+{
+    __shared__ double2 buf[2048];   // 32 KB
+    for (int i = threadIdx.x; i < 2048; i += 256) buf[i] = make_double2(i, -i);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    double2 acc = make_double2(0, 0);
+    int off = bcast ? 0 : lane;
+#pragma unroll 8
+    for (int it = 0; it < ITERS; ++it) {
+        const double2 v = buf[(off + it * 64) & 2047];
+        acc.x += v.x; acc.y += v.y;
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = acc.x + acc.y;
+}
+
+__global__ __launch_bounds__(256) void k_bperm(double *out)
+{
+    const int lane = threadIdx.x & 63;
+    int v = lane * 3 + blockIdx.x;
+    int acc = 0;
+#pragma unroll 8
+    for (int it = 0; it < ITERS; ++it) {
+        acc += __builtin_amdgcn_ds_bpermute(((lane + it) & 63) * 4, v + it);
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_glob(const double *__restrict__ src, double *out)
+{
+    // src holds 16 KB per block (L1-resident); each lane reads W doubles
+    const int lane = threadIdx.x & 63;
+    const double *base = src + (blockIdx.x & 1023) * 2048;
+    double acc = 0;
+#pragma unroll 8
+    for (int it = 0; it < ITERS; ++it) {
+        const int o = ((lane + it * 64) * W) & 2047;
+        if constexpr (W == 2) {
+            const double2 v = *reinterpret_cast<const double2 *>(base + o);
+            acc += v.x + v.y;
+        } else {
+            acc += base[o];
+        }
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+int main()
+{
+    int dev = 0; hipDeviceProp_t p; CHECK(hipGetDeviceProperties(&p, dev));
+    const int cus = p.multiProcessorCount;
+    const double ghz = p.clockRate / 1e6;
+    printf("CUs %d clock %.3f GHz LDS/block %zu\n", cus, ghz, p.sharedMemPerBlock);
+    const int blocks = cus * 8;
+    double *out, *src;
+    CHECK(hipMalloc(&out, sizeof(double) * blocks * 256));
+    CHECK(hipMalloc(&src, sizeof(double) * 1024 * 2048));
+    CHECK(hipMemset(src, 0, sizeof(double) * 1024 * 2048));
+    hipEvent_t a, b; CHECK(hipEventCreate(&a)); CHECK(hipEventCreate(&b));
+    auto run = [&](const char *name, double bytes_per_lane_iter, auto launch) {
+        launch(); CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(a)); for (int r = 0; r < 5; ++r) launch(); CHECK(hipEventRecord(b));
+        CHECK(hipEventSynchronize(b)); float ms; CHECK(hipEventElapsedTime(&ms, a, b)); ms /= 5;
+        const double bytes = (double)blocks * 256 * ITERS * bytes_per_lane_iter;
+        const double per_cu_clk = bytes / (ms * 1e-3) / cus / (ghz * 1e9);
+        const double instr = (double)blocks * 4 * ITERS;   // wave instructions
+        printf("%-22s %8.3f ms  %8.1f TB/s  %6.1f B/clk/CU  %6.2f clk/wave-instr/CU\n", name, ms,
+               bytes / (ms * 1e-3) / 1e12, per_cu_clk, (ms * 1e-3) * ghz * 1e9 * cus / instr);
+    };
+    run("lds_b128_distinct", 16, [&] { k_lds<<<blocks, 256>>>(out, 0); });
+    run("lds_b128_broadcast", 16, [&] { k_lds<<<blocks, 256>>>(out, 1); });
+    run("ds_bpermute_b32", 4, [&] { k_bperm<<<blocks, 256>>>(out); });
+    run("global_dwordx4_L1", 16, [&] { k_glob<2><<<blocks, 256>>>(src, out); });
+    run("global_dwordx2_L1", 8, [&] { k_glob<1><<<blocks, 256>>>(src, out); });
+    CHECK(hipFree(out)); CHECK(hipFree(src));
+    return 0;
+}
