@@ -55,7 +55,7 @@ def parse_args():
     ap.add_argument("--nodes", type=int, default=40962)
     ap.add_argument("--members", type=int, default=1)
     ap.add_argument("--member-block", type=int, default=0)
-    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw", "pair"])
     ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--order", default="host", choices=["host", "morton"])
     ap.add_argument("--tiled", action="store_true")
@@ -121,8 +121,10 @@ def run_reference_cpu(path: str, n_nodes: int, target_s: float):
 def dominant_kernel(args):
     """Name of the stage kernel the options select (the one timed)."""
     if args.members == 1:
-        return {"auto": "k_stage_quadw", "quadw": "k_stage_quadw", "quad": "k_stage_quad",
-                "lane": "k_stage_lane"}[args.kernel]
+        if args.kernel == "auto":
+            return "k_stage_pair" if args.nodes >= 100000 else "k_stage_quadw"
+        return {"quadw": "k_stage_quadw", "quad": "k_stage_quad",
+                "lane": "k_stage_lane", "pair": "k_stage_pair"}[args.kernel]
     if args.tiled:
         return "k_stage_tmem" if args.member_block == 32 else "k_stage_tiled"
     return "k_stage_mpair" if args.member_block in (0, 64) and args.members % 64 == 0 \

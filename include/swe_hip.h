@@ -86,12 +86,16 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
  * share, when not NULL, is a context of the same mesh whose device copy of
  * the static data is reused (s may then be NULL): the static data stay
  * resident until the last context using them is destroyed. */
-#define SWE_HIP_KERNEL_AUTO 0   /* single state: QUADW                       */
+#define SWE_HIP_KERNEL_AUTO 0   /* single state: PAIR for N >= 100000 with a
+                                   31-point stencil, else QUADW              */
 #define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
 #define SWE_HIP_KERNEL_LANE 2   /* single state: 1 lane per node             */
 #define SWE_HIP_KERNEL_QUADW 3  /* single state: 4 lanes per node, weights,
                                    ids, geometry loaded once per quad and
                                    shared by DPP                             */
+#define SWE_HIP_KERNEL_PAIR 4   /* single state: 2 lanes per node, 16-byte
+                                   weight and state loads (31-point stencils;
+                                   other widths use QUADW)                   */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 typedef struct swe_hip_options {

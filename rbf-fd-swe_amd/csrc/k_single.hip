@@ -1,4 +1,4 @@
-// Launchers of the single-state stage kernels (quad and lane mappings).
+// Launchers of the single-state stage kernels (quad, quadw, pair, lane mappings).
 #include "swe_ctx.h"
 
 namespace swe {
@@ -43,6 +43,17 @@ void launch_quadw(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
         k_stage_quadw<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
 }
 
+template <int STAGE, bool EX, int NN>
+void launch_pair(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    if constexpr (NN > 0) {
+        st.nblk = blocks_x(c, 2L * c->n);
+        k_stage_pair<STAGE, EX, NN><<<dim3(st.nblk), dim3(kBlock), 0, c->stream>>>(m, st);
+    } else {
+        launch_quadw<STAGE, EX, NN>(c, m, st);   // needs a compile-time stencil width
+    }
+}
+
 }  // namespace
 
 template <int STAGE, bool EX, int NN>
@@ -53,6 +64,8 @@ void family_single(const swe_hip_ctx *c, const DevStage &st)
         launch_lane<STAGE, EX, NN>(c, m, st);
     else if (c->kernel == SWE_HIP_KERNEL_QUADW)
         launch_quadw<STAGE, EX, NN>(c, m, st);
+    else if (c->kernel == SWE_HIP_KERNEL_PAIR)
+        launch_pair<STAGE, EX, NN>(c, m, st);
     else
         launch_quad<STAGE, EX, NN>(c, m, st);
 }

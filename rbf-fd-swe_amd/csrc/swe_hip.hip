@@ -510,7 +510,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "static data is NULL");
     if (o->members < 1 || o->members > 4096)
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
-    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_QUADW)
+    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_PAIR)
         return fail(SWE_HIP_EINVAL, "kernel must be one of SWE_HIP_KERNEL_*");
     if (o->prefetch != 0 && o->prefetch != 4 && o->prefetch != 8)
         return fail(SWE_HIP_EINVAL, "prefetch must be 0, 4 or 8");
@@ -561,8 +561,14 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->d_geo = m->d_geo;
     c->B = o->members;
     c->MB = mb;
-    c->kernel = (o->kernel == SWE_HIP_KERNEL_LANE || o->kernel == SWE_HIP_KERNEL_QUAD)
-                    ? o->kernel : SWE_HIP_KERNEL_QUADW;
+    // auto: the pair mapping halves the address-unit work per node and wins
+    // once the mesh fills the chip (C3: +5 %); quadw has twice the waves
+    // for smaller meshes.
+    if (o->kernel == SWE_HIP_KERNEL_AUTO)
+        c->kernel = (m->n >= 100000 && m->nnbr == 31) ? SWE_HIP_KERNEL_PAIR
+                                                      : SWE_HIP_KERNEL_QUADW;
+    else
+        c->kernel = o->kernel;
     c->prefetch = o->prefetch;
     c->flags = o->flags;
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) !=

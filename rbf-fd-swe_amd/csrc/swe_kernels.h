@@ -188,6 +188,34 @@ __device__ __forceinline__ void stage_update(const DevStage &st, long e, double 
     }
 }
 
+// Stage update of two adjacent elements (e, e + 1; e even: two members, or
+// two variables of one node) with 16-byte loads and stores; per element the
+// arithmetic of stage_update.
+template <int STAGE, bool EXACT>
+__device__ __forceinline__ void stage_update_pair(const DevStage &st, long e, double F0, double F1)
+{
+    double2 *out = reinterpret_cast<double2 *>(st.out + e);
+    if constexpr (STAGE == ST_RHS) {
+        *out = make_double2(F0, F1);
+    } else if constexpr (STAGE == ST_0) {
+        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
+        *reinterpret_cast<double2 *>(st.d + e) = make_double2(F0, F1);
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, F0), axpy<EXACT>(hv.y, st.c, F1));
+    } else if constexpr (STAGE == ST_1 || STAGE == ST_2) {
+        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
+        double2 *dp = reinterpret_cast<double2 *>(st.d + e);
+        const double2 d0 = *dp;
+        *dp = make_double2(axpy<EXACT>(d0.x, st.dcoef, F0), axpy<EXACT>(d0.y, st.dcoef, F1));
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, F0), axpy<EXACT>(hv.y, st.c, F1));
+    } else {
+        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
+        const double2 d0 = *reinterpret_cast<const double2 *>(st.d + e);
+        const double dv0 = axpy<EXACT>(d0.x, st.dcoef, F0);
+        const double dv1 = axpy<EXACT>(d0.y, st.dcoef, F1);
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, dv0), axpy<EXACT>(hv.y, st.c, dv1));
+    }
+}
+
 // Broadcast lane K of every aligned group of 4 lanes (DPP quad_perm).
 template <int K>
 __device__ __forceinline__ double quad_bcast(double v)
@@ -423,6 +451,116 @@ __global__ __launch_bounds__(256) void k_stage_quadw(DevMesh m, DevStage st)
         stage_update<STAGE, EXACT>(st, 4L * node + k, Fk);
 }
 
+// DPP within aligned lane pairs: swap the two lanes, or broadcast lane K.
+__device__ __forceinline__ double pair_swap(double v)
+{
+    constexpr int ctrl = 0xB1;  // quad_perm [1, 0, 3, 2]
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int K>
+__device__ __forceinline__ int pair_bcast_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, K ? 0xF5 : 0xA0, 0xF, 0xF, false);  // [K, K, 2+K, 2+K]
+}
+template <int K>
+__device__ __forceinline__ double pair_bcast(double v)
+{
+    constexpr int ctrl = K ? 0xF5 : 0xA0;
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// ---------------------------------------------------------------------------
+// Single state (B == 1), pair mapping: two lanes per node.  Lane k owns
+// variables 2k, 2k+1 and weights 2k, 2k+1 of every slot: per slot it issues
+// one 16-byte weight load, one 16-byte gather (the two variables of the
+// neighbour) and every other slot's stencil id, and takes the other two
+// weights from its partner by DPP -- 2.5 global load instructions per slot
+// for 32 nodes, where the quad mapping needs 4.5.  A lane keeps 8 sequential
+// sums: its two variables against its own two weights and against the
+// partner's; the 16 sums, the own state and the geometry are exchanged by
+// DPP once per node, each lane evaluates the RHS and updates its two
+// variables with one 16-byte access.  Same sums, same order as the other
+// single-state kernels (bitwise in exact mode).
+template <int STAGE, bool EXACT, int NNBR>
+__global__ __launch_bounds__(256) void k_stage_pair(DevMesh m, DevStage st)
+{
+    static_assert(NNBR > 0, "compile-time stencil width");
+    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int node_raw = t >> 1;
+    const int k = t & 1;
+    const bool live = node_raw < m.n;
+    const int node = live ? node_raw : m.n - 1;
+    const long ld = slot_stride(m);
+    const long sb = slot_base<NNBR>(m, node);
+    const int *__restrict__ nb = m.nbr + sb;
+    const double *__restrict__ wp = m.wts + 4L * sb + 2 * k;
+    const double *__restrict__ src = st.src;
+
+    constexpr int NQ = (NNBR + 1) / 2;
+    int qv[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int j = 2 * i + k;
+        qv[i] = nb[(j < NNBR ? j : 0) * ld];
+    }
+    // own0/own1: this lane's weights (2k, 2k+1); oth0/oth1: the partner's
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;   // variable 2k   x (own0, own1, oth0, oth1)
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;   // variable 2k+1 x (own0, own1, oth0, oth1)
+    static_for<NNBR>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int q = pair_bcast_i<j & 1>(qv[j >> 1]);
+        const double2 w = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        const double2 s = *reinterpret_cast<const double2 *>(src + 4L * q + 2 * k);
+        const double o0 = pair_swap(w.x), o1 = pair_swap(w.y);
+        a0 = madd<EXACT>(a0, w.x, s.x); a1 = madd<EXACT>(a1, w.y, s.x);
+        a2 = madd<EXACT>(a2, o0, s.x);  a3 = madd<EXACT>(a3, o1, s.x);
+        b0 = madd<EXACT>(b0, w.x, s.y); b1 = madd<EXACT>(b1, w.y, s.y);
+        b2 = madd<EXACT>(b2, o0, s.y);  b3 = madd<EXACT>(b3, o1, s.y);
+    });
+    // (dx, dy, dz, l) of this lane's variables: lane 0 owns (Dx, Dy), lane 1 (Dz, L)
+    const double ax = k ? a2 : a0, ay = k ? a3 : a1, az = k ? a0 : a2, al = k ? a1 : a3;
+    const double bx = k ? b2 : b0, by = k ? b3 : b1, bz = k ? b0 : b2, bl = k ? b1 : b3;
+    // partner's variables
+    const double pax = pair_swap(ax), pay = pair_swap(ay), paz = pair_swap(az), pal = pair_swap(al);
+    const double pbx = pair_swap(bx), pby = pair_swap(by), pbz = pair_swap(bz), pbl = pair_swap(bl);
+    Sums S;
+    // variables u, v live on lane 0 (a, b); w, h on lane 1
+    S.dudx = k ? pax : ax; S.dudy = k ? pay : ay; S.dudz = k ? paz : az; S.lu = k ? pal : al;
+    S.dvdx = k ? pbx : bx; S.dvdy = k ? pby : by; S.dvdz = k ? pbz : bz; S.lv = k ? pbl : bl;
+    S.dwdx = k ? ax : pax; S.dwdy = k ? ay : pay; S.dwdz = k ? az : paz; S.lw = k ? al : pal;
+    S.dhdx = k ? bx : pbx; S.dhdy = k ? by : pby; S.dhdz = k ? bz : pbz; S.lh = k ? bl : pbl;
+
+    const double2 own = *reinterpret_cast<const double2 *>(src + 4L * node + 2 * k);
+    const double po0 = pair_swap(own.x), po1 = pair_swap(own.y);
+    const double u = k ? po0 : own.x, v = k ? po1 : own.y;
+    const double w = k ? own.x : po0, h = k ? own.y : po1;
+    double gv[(GEO_N + 1) / 2];
+#pragma unroll
+    for (int r = 0; r < (GEO_N + 1) / 2; ++r) {
+        const int f = 2 * r + k;
+        gv[r] = m.geo[(long)(f < GEO_N ? f : 0) * m.ld + node];
+    }
+    double gf[GEO_N];
+    static_for<GEO_N>([&](auto fc) {
+        constexpr int f = decltype(fc)::value;
+        gf[f] = pair_bcast<f & 1>(gv[f >> 1]);
+    });
+    Geo g;
+    g.x = gf[G_X]; g.y = gf[G_Y]; g.z = gf[G_Z]; g.f = gf[G_F]; g.ghm = gf[G_GHM];
+    g.pu0 = gf[G_PU]; g.pu1 = gf[G_PU + 1]; g.pu2 = gf[G_PU + 2];
+    g.pv0 = gf[G_PV]; g.pv1 = gf[G_PV + 1]; g.pv2 = gf[G_PV + 2];
+    g.pw0 = gf[G_PW]; g.pw1 = gf[G_PW + 1]; g.pw2 = gf[G_PW + 2];
+    g.gx = gf[G_GG]; g.gy = gf[G_GG + 1]; g.gz = gf[G_GG + 2];
+    double F[4];
+    rhs_point(S, u, v, w, h, g, m.gh0, F);
+    if (live)
+        stage_update_pair<STAGE, EXACT>(st, 4L * node + 2 * k, k ? F[2] : F[0], k ? F[3] : F[1]);
+}
+
 struct LaneSlot {
     double2 w01, w23, s01, s23;
 };
@@ -607,33 +745,6 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
 #pragma unroll
         for (int var = 0; var < 4; ++var)
             stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
-    }
-}
-
-// Stage update of two adjacent members (elements e, e + 1; e even) with
-// 16-byte loads and stores; per element the arithmetic of stage_update.
-template <int STAGE, bool EXACT>
-__device__ __forceinline__ void stage_update_pair(const DevStage &st, long e, double F0, double F1)
-{
-    double2 *out = reinterpret_cast<double2 *>(st.out + e);
-    if constexpr (STAGE == ST_RHS) {
-        *out = make_double2(F0, F1);
-    } else if constexpr (STAGE == ST_0) {
-        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
-        *reinterpret_cast<double2 *>(st.d + e) = make_double2(F0, F1);
-        *out = make_double2(axpy<EXACT>(hv.x, st.c, F0), axpy<EXACT>(hv.y, st.c, F1));
-    } else if constexpr (STAGE == ST_1 || STAGE == ST_2) {
-        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
-        double2 *dp = reinterpret_cast<double2 *>(st.d + e);
-        const double2 d0 = *dp;
-        *dp = make_double2(axpy<EXACT>(d0.x, st.dcoef, F0), axpy<EXACT>(d0.y, st.dcoef, F1));
-        *out = make_double2(axpy<EXACT>(hv.x, st.c, F0), axpy<EXACT>(hv.y, st.c, F1));
-    } else {
-        const double2 hv = *reinterpret_cast<const double2 *>(st.h + e);
-        const double2 d0 = *reinterpret_cast<const double2 *>(st.d + e);
-        const double dv0 = axpy<EXACT>(d0.x, st.dcoef, F0);
-        const double dv1 = axpy<EXACT>(d0.y, st.dcoef, F1);
-        *out = make_double2(axpy<EXACT>(hv.x, st.c, dv0), axpy<EXACT>(hv.y, st.c, dv1));
     }
 }
 

@@ -147,7 +147,8 @@ def test_tiled_fast_composition_and_active(pkg, mb):
 
 @pytest.mark.parametrize("kernel,pf,sblk", [("quad", 0, 0), ("quad", 8, 0), ("lane", 0, 0),
                                            ("lane", 4, 0), ("quad", 0, 16), ("lane", 4, 16),
-                                           ("quadw", 0, 0), ("quadw", 4, 16)])
+                                           ("quadw", 0, 0), ("quadw", 4, 16), ("pair", 0, 0),
+                                           ("pair", 0, 16)])
 def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk):
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
     print(f"case: single-state {kernel} prefetch={pf} slot_block={sblk} morton exact", flush=True)
@@ -157,6 +158,20 @@ def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk):
         ctx.set_state(prob.H_init)
         ctx.rk_steps(3)
         assert np.array_equal(ctx.get_state(), want)
+
+
+@pytest.mark.parametrize("kernel,order,sblk", [("pair", "host", 0), ("pair", "morton", 16),
+                                               ("quadw", "morton", 16), ("lane", "host", 0)])
+def test_single_state_variants_fast(pkg, pyoracle, kernel, order, sblk):
+    """Fast single-state kernels within 1e-12 of the oracle after 50 steps."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
+    print(f"case: single-state fast {kernel} {order} slot_block={sblk} 50 steps", flush=True)
+    want = pyoracle.Oracle(prob).run(prob.H_init, 50)
+    with pkg.runtime.HipContext(prob, kernel=kernel, order=order, slot_block=sblk) as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(50)
+        err = rel_by_var(ctx.get_state(), want, prob)
+        assert (err <= 1e-12).all(), err
 
 
 @pytest.mark.parametrize("tiled,mb", [(False, 64), (False, 8), (True, 8), (True, 32)])
