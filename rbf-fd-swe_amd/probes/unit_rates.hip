@@ -1,7 +1,8 @@
 // unit_rates: per-CU throughput of the memory paths the stage kernels choose
 // between on gfx950 -- LDS b128 reads (distinct and broadcast addresses),
-// ds_bpermute, and L1-resident global loads of 16 and 8 bytes per lane.
-// Prints bytes per clock per CU and clocks per wave64 instruction per CU.
+// ds_bpermute, L1-resident global loads of 16 and 8 bytes per lane, and the
+// FP64 FMA rate.  Prints bytes per clock per CU and clocks per wave64
+// instruction per CU (per SIMD for the FMA).
 // All indices stay inside their buffers: LDS offsets are masked to 32 KB,
 // global offsets to 16 KB per workgroup.
 // Build: make -C rbf-fd-swe_amd probes; run: rbf-fd-swe_amd/probes/unit_rates
@@ -38,6 +39,21 @@ __global__ __launch_bounds__(256) void k_bperm(double *out)
     for (int it = 0; it < ITERS; ++it) {
         acc += __builtin_amdgcn_ds_bpermute(((lane + it) & 63) * 4, v + it);
     }
+    out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_fma64(double *out, double a)
+{
+    double x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = threadIdx.x + i;
+    for (int it = 0; it < ITERS; ++it) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = __builtin_fma(x[i], a, 1.0e-9);
+    }
+    double acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc += x[i];
     out[blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
@@ -88,6 +104,17 @@ int main()
     run("ds_bpermute_b32", 4, [&] { k_bperm<<<blocks, 256>>>(out); });
     run("global_dwordx4_L1", 16, [&] { k_glob<2><<<blocks, 256>>>(src, out); });
     run("global_dwordx2_L1", 8, [&] { k_glob<1><<<blocks, 256>>>(src, out); });
+    // FP64 FMA: 8 independent chains per lane, ITERS x 8 FMAs = 16 flop each
+    {
+        k_fma64<<<blocks, 256>>>(out, 0.999999); CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(a)); for (int r = 0; r < 5; ++r) k_fma64<<<blocks, 256>>>(out, 0.999999);
+        CHECK(hipEventRecord(b)); CHECK(hipEventSynchronize(b));
+        float ms; CHECK(hipEventElapsedTime(&ms, a, b)); ms /= 5;
+        const double fmas = (double)blocks * 256 * ITERS * 8;
+        const double instr = (double)blocks * 4 * ITERS * 8;
+        printf("%-22s %8.3f ms  %8.1f TFLOP/s  %6.2f clk/wave-instr/SIMD\n", "v_fma_f64", ms,
+               2 * fmas / (ms * 1e-3) / 1e12, (ms * 1e-3) * ghz * 1e9 * cus * 4 / instr);
+    }
     CHECK(hipFree(out)); CHECK(hipFree(src));
     return 0;
 }
