@@ -270,14 +270,15 @@ __device__ __forceinline__ Geo load_geo_quad(const double *__restrict__ geo, int
 }
 
 // Stencil-slot loop shared by the stage kernels.  PF == 0 leaves the load
-// schedule to the compiler.  PF = D > 0 software-pipelines it: all neighbour
-// ids are loaded first, then a ring of D slots is kept in flight -- each
-// iteration consumes the oldest slot and issues the loads of slot j + D --
-// with scheduling barriers between iterations, so a wave always has D
-// gathers and 2*D weight loads outstanding and the compiler can neither
-// hoist further loads nor (in exact mode) precompute all 31x16 products
-// ahead of the serial sums, which would exhaust the registers.
-// Accumulation is sequential in j for every sum in all forms.
+// schedule to the compiler.  PF = D > 0 writes the loop as a ring of D slots:
+// all neighbour ids are loaded first, each iteration consumes the oldest
+// slot and issues the loads of slot j + D, with scheduling barriers between
+// iterations.  The barriers carry no data dependency, so they shape the
+// compiler's schedule without fixing it (in exact mode they keep it from
+// precomputing all 31x16 products ahead of the serial sums); the kernels
+// that need a firm rolling window (k_stage_mpair, k_stage_tmem) instead tie
+// each next address to the running sums with an empty asm.  Accumulation
+// is sequential in j for every sum in all forms.
 template <int NNBR, int PF, typename LoadFn, typename AccFn, typename Slot>
 __device__ __forceinline__ void stencil_loop(const int *__restrict__ nb, long ld, int nn,
                                              LoadFn load, AccFn acc, Slot *)
@@ -621,7 +622,8 @@ __global__ __launch_bounds__(256) void k_stage_lane(DevMesh m, DevStage st)
 // member; the stencil weights and ids are shared by the MB lanes of a node
 // (wave-uniform scalar loads when MB == 64), so one weight read from HBM
 // serves every member of the block.  The neighbour state of a variable is a
-// contiguous run of MB doubles.
+// contiguous run of MB doubles.  Member blocks below 64, and 64 with a
+// run-time stencil width (31-point stencils at 64 use k_stage_mpair).
 struct MemberSlot {
     double2 w01, w23;
     double su, sv, sw, sh;
@@ -649,92 +651,31 @@ __global__ __launch_bounds__(256) void k_stage_members(DevMesh m, DevStage st)
     const double *__restrict__ src = st.src + mem;
 
     Sums S = {};
-    if constexpr (MB == 64 && !EXACT && NNBR > 0 && NNBR <= 32) {
-        // One node per wave: lane j fetches the id of slot j and lanes 2j,
-        // 2j+1 fetch its (Dx, Dy) and (Dz, L), one vector load each, up
-        // front.  The weights go to a wave-private LDS row and are read back
-        // per slot with uniform-address (broadcast) LDS loads; the id moves
-        // to a scalar register with v_readlane.  PF > 0: the neighbour
-        // gathers run as a rolling pipeline PF slots deep (slot j + PF is
-        // issued right after slot j is consumed); PF == 0 leaves the order
-        // to the compiler, which issues them in batches of 3-4 slots.
-        constexpr int D = PF > 0 ? PF : 6;
-        __shared__ double2 wrow[4][64];
-        const int lane = (int)threadIdx.x & 63;
-        const int wid = (int)threadIdx.x >> 6;
-        const int jq = lane < NNBR ? lane : 0;
-        const int jw = (lane >> 1) < NNBR ? (lane >> 1) : 0;
-        const int qv = nb[jq * ld];
-        wrow[wid][lane] = *reinterpret_cast<const double2 *>(wp + 4L * jw * ld + 2 * (lane & 1));
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double ru[D], rv[D], rw[D], rh[D];
-        auto issue = [&](int j, int q) {
-            const double *sq = src + 4L * q * B;
-            ru[j % D] = sq[0];
-            rv[j % D] = sq[B];
-            rw[j % D] = sq[2 * B];
-            rh[j % D] = sq[3 * B];
-        };
-#pragma unroll
-        for (int j = 0; j < D && j < NNBR; ++j)
-            issue(j, __builtin_amdgcn_readlane(qv, j));
-#pragma unroll
-        for (int j = 0; j < NNBR; ++j) {
-            const double su = ru[j % D], sv = rv[j % D], sw = rw[j % D], sh = rh[j % D];
-            const double2 w01 = wrow[wid][2 * j];
-            const double2 w23 = wrow[wid][2 * j + 1];
-            S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
-            S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
-            S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
-            S.dwdy = madd<EXACT>(S.dwdy, w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, w01.y, sh);
-            S.dudz = madd<EXACT>(S.dudz, w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, w23.x, sv);
-            S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
-            S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
-            S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
-            if (j + D < NNBR) {
-                // Empty asm: ties slot j + D's id to all 16 sums after slot
-                // j, so slot j is fully consumed (its registers free) before
-                // the gathers of slot j + D issue.  (sched_barrier cannot
-                // hold this order: it carries no dependency and is hoisted.)
-                int q = __builtin_amdgcn_readlane(qv, j + D);
-                if constexpr (PF > 0)
-                    asm volatile(""
-                                 : "+s"(q)
-                                 : "v"(S.dudx), "v"(S.dvdx), "v"(S.dwdx), "v"(S.dhdx),
-                                   "v"(S.dudy), "v"(S.dvdy), "v"(S.dwdy), "v"(S.dhdy),
-                                   "v"(S.dudz), "v"(S.dvdz), "v"(S.dwdz), "v"(S.dhdz),
-                                   "v"(S.lu), "v"(S.lv), "v"(S.lw), "v"(S.lh));
-                issue(j + D, q);
-            }
-        }
-    } else {
-        auto load = [&](int j, int q) {
-            if constexpr (MB == 64)
-                q = __builtin_amdgcn_readfirstlane(q);
-            MemberSlot r;
-            r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
-            r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
-            const double *sq = src + 4L * q * B;
-            r.su = sq[0];
-            r.sv = sq[B];
-            r.sw = sq[2 * B];
-            r.sh = sq[3 * B];
-            return r;
-        };
-        auto acc = [&](const MemberSlot &r) {
-            const double su = r.su, sv = r.sv, sw = r.sw, sh = r.sh;
-            S.dudx = madd<EXACT>(S.dudx, r.w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, r.w01.x, sv);
-            S.dwdx = madd<EXACT>(S.dwdx, r.w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, r.w01.x, sh);
-            S.dudy = madd<EXACT>(S.dudy, r.w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, r.w01.y, sv);
-            S.dwdy = madd<EXACT>(S.dwdy, r.w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, r.w01.y, sh);
-            S.dudz = madd<EXACT>(S.dudz, r.w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, r.w23.x, sv);
-            S.dwdz = madd<EXACT>(S.dwdz, r.w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, r.w23.x, sh);
-            S.lu = madd<EXACT>(S.lu, r.w23.y, su); S.lv = madd<EXACT>(S.lv, r.w23.y, sv);
-            S.lw = madd<EXACT>(S.lw, r.w23.y, sw); S.lh = madd<EXACT>(S.lh, r.w23.y, sh);
-        };
-        stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (MemberSlot *)nullptr);
-    }
+    auto load = [&](int j, int q) {
+        if constexpr (MB == 64)
+            q = __builtin_amdgcn_readfirstlane(q);
+        MemberSlot r;
+        r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
+        const double *sq = src + 4L * q * B;
+        r.su = sq[0];
+        r.sv = sq[B];
+        r.sw = sq[2 * B];
+        r.sh = sq[3 * B];
+        return r;
+    };
+    auto acc = [&](const MemberSlot &r) {
+        const double su = r.su, sv = r.sv, sw = r.sw, sh = r.sh;
+        S.dudx = madd<EXACT>(S.dudx, r.w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, r.w01.x, sv);
+        S.dwdx = madd<EXACT>(S.dwdx, r.w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, r.w01.x, sh);
+        S.dudy = madd<EXACT>(S.dudy, r.w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, r.w01.y, sv);
+        S.dwdy = madd<EXACT>(S.dwdy, r.w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, r.w01.y, sh);
+        S.dudz = madd<EXACT>(S.dudz, r.w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, r.w23.x, sv);
+        S.dwdz = madd<EXACT>(S.dwdz, r.w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, r.w23.x, sh);
+        S.lu = madd<EXACT>(S.lu, r.w23.y, su); S.lv = madd<EXACT>(S.lv, r.w23.y, sv);
+        S.lw = madd<EXACT>(S.lw, r.w23.y, sw); S.lh = madd<EXACT>(S.lh, r.w23.y, sh);
+    };
+    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (MemberSlot *)nullptr);
     const double *own = src + 4L * node * B;
     const double u = own[0], v = own[B], w = own[2 * B], h = own[3 * B];
     const Geo g = load_geo(m.geo, m.ld, node);

@@ -1,7 +1,7 @@
 // unit_rates: per-CU throughput of the memory paths the stage kernels choose
 // between on gfx950 -- LDS b128 reads (distinct and broadcast addresses),
 // ds_bpermute, L1-resident global loads of 16 and 8 bytes per lane, and the
-// FP64 FMA rate.  Prints bytes per clock per CU and clocks per wave64
+// FP64 FMA and FP64 MFMA (16x16x4) rates.  Prints bytes per clock per CU and clocks per wave64
 // instruction per CU (per SIMD for the FMA).
 // All indices stay inside their buffers: LDS offsets are masked to 32 KB,
 // global offsets to 16 KB per workgroup.
@@ -55,6 +55,22 @@ __global__ __launch_bounds__(256) void k_fma64(double *out, double a)
 #pragma unroll
     for (int i = 0; i < 8; ++i) acc += x[i];
     out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_mfma64(double *out, double a)
+{
+    // 4 independent 16x16x4 FP64 accumulations per wave
+    mfma_d4 c0 = {0, 0, 0, 0}, c1 = c0, c2 = c0, c3 = c0;
+    const double x = a + threadIdx.x * 1e-9, y = a - threadIdx.x * 1e-9;
+    for (int it = 0; it < ITERS; ++it) {
+        c0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f64_16x16x4f64(y, x, c1, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, c2, 0, 0, 0);
+        c3 = __builtin_amdgcn_mfma_f64_16x16x4f64(y, y, c3, 0, 0, 0);
+    }
+    const mfma_d4 s = c0 + c1 + c2 + c3;
+    out[blockIdx.x * 256 + threadIdx.x] = s.x + s.y + s.z + s.w;
 }
 
 template <int W>
@@ -114,6 +130,16 @@ int main()
         const double instr = (double)blocks * 4 * ITERS * 8;
         printf("%-22s %8.3f ms  %8.1f TFLOP/s  %6.2f clk/wave-instr/SIMD\n", "v_fma_f64", ms,
                2 * fmas / (ms * 1e-3) / 1e12, (ms * 1e-3) * ghz * 1e9 * cus * 4 / instr);
+    }
+    // FP64 MFMA 16x16x4: 2*16*16*4 flop per wave instruction
+    {
+        k_mfma64<<<blocks, 256>>>(out, 0.5); CHECK(hipDeviceSynchronize());
+        CHECK(hipEventRecord(a)); for (int r = 0; r < 5; ++r) k_mfma64<<<blocks, 256>>>(out, 0.5);
+        CHECK(hipEventRecord(b)); CHECK(hipEventSynchronize(b));
+        float ms; CHECK(hipEventElapsedTime(&ms, a, b)); ms /= 5;
+        const double instr = (double)blocks * 4 * ITERS * 4;
+        printf("%-22s %8.3f ms  %8.1f TFLOP/s  %6.2f clk/wave-instr/SIMD\n", "v_mfma_f64_16x16x4", ms,
+               instr * 2 * 16 * 16 * 4 / (ms * 1e-3) / 1e12, (ms * 1e-3) * ghz * 1e9 * cus * 4 / instr);
     }
     CHECK(hipFree(out)); CHECK(hipFree(src));
     return 0;
