@@ -58,7 +58,7 @@ struct SweTiles {
     int *d_tile_start = nullptr, *d_ustart = nullptr, *d_ulist = nullptr;
     unsigned short *d_lidx = nullptr;
     unsigned short *d_lidx_node = nullptr;   // [n][32] node-major copy of d_lidx
-    int *d_ulist_tile = nullptr;             // [ntiles][kTmemRows] (member-lane kernel)
+    int *d_ulist_tile = nullptr;             // [ntiles][kTmemStride] (member-lane kernel)
 };
 
 struct swe_hip_ctx {

@@ -449,10 +449,10 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
     // fixed-stride copy for the member-lane kernel (tiles of <= kTmemRows rows)
     std::vector<int> ulist_tile;
     if (umax <= kTmemRows) {
-        ulist_tile.assign((size_t)tile * kTmemRows, 0);
+        ulist_tile.assign((size_t)tile * kTmemStride, 0);
         for (int k = 0; k < tile; ++k)
             std::copy(ulist.begin() + ustart[k], ulist.begin() + ustart[k + 1],
-                      ulist_tile.begin() + (size_t)k * kTmemRows);
+                      ulist_tile.begin() + (size_t)k * kTmemStride);
     }
     SweTiles &t = c->tiles;
     t.ntiles = tile;

@@ -826,7 +826,7 @@ struct DevTiles {
     const int *tile_start;          // [ntiles + 1] first node of each tile
     const int *ustart;              // [ntiles + 1] offsets into ulist
     const int *ulist;               // distinct stencil nodes of each tile, ascending
-    const int *ulist_tile;          // the same at a fixed stride: [ntiles][kTmemRows],
+    const int *ulist_tile;          // the same at a fixed stride: [ntiles][kTmemStride],
                                     // zero-padded (member-lane kernel only)
     const unsigned short *lidx;     // [nnbr][ld] position of nbr(j, node) in its tile's list
     const unsigned short *lidx_node;  // [n][32] the same, node-major (64 B per node)
@@ -967,10 +967,11 @@ __device__ __forceinline__ Geo load_geo_pair(const double *__restrict__ geo, int
 // keeps reads contiguous across lanes.  Per member the sums, the RHS and
 // the stage update are those of k_stage_members (bitwise in exact mode).
 constexpr int kTmemTile = 8;    // nodes per tile: 4 waves x 2 nodes
-constexpr int kTmemRows = 64;   // distinct rows per tile: 64 KB of LDS
+constexpr int kTmemRows = 70;   // distinct rows per tile: 70 KB of LDS (+10 KB: 2 workgroups fill 160 KB)
 constexpr int kTmemWtsDoubles = 4 * 2 * 32 * 4;   // weights [wave][node][slot][4]
 constexpr int kTmemGeoDoubles = 4 * 2 * 32;       // geometry [wave][node][32]
-constexpr int kTmemFill = kTmemRows / 4;          // rows per wave
+constexpr int kTmemFill = (kTmemRows + 3) / 4;    // rows per wave
+constexpr int kTmemStride = 4 * kTmemFill;        // per-tile stride of the fixed-stride row ids
 
 template <int STAGE, bool EXACT>
 __device__ __forceinline__ void stage_update_pre(const DevStage &st, long e, double Fv,
@@ -1073,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_stage_tmem(DevMesh m, DevTiles tl, DevS
         // tile: one merged scalar load of ids at a fixed per-tile stride (no
         // wait for the tile's offset); the zero padding past the tile's
         // count is a readable node id, loaded but not stored
-        const int *ul = tl.ulist_tile + (long)I.tile * kTmemRows + wid * kTmemFill;
+        const int *ul = tl.ulist_tile + (long)I.tile * kTmemStride + wid * kTmemFill;
 #pragma unroll
         for (int k = 0; k < kTmemFill; ++k)
             L.fx[k] = *reinterpret_cast<const double2 *>(s0 + 4L * c_i32(ul + k) * B);
