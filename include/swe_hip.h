@@ -82,10 +82,12 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
 
 /* Extended creation.  members/flags/device as swe_hip_create; member_block
  * is the number of members one lane group of the ensemble kernel serves
- * (a power of two 2..64 dividing members; 0 = the largest such value);
+ * (a power of two 2..64 dividing members; 0 = the largest such value; 64
+ * with a 31-point stencil = the member-pair kernel, two members per lane);
  * share, when not NULL, is a context of the same mesh whose device copy of
  * the static data is reused (s may then be NULL): the static data stay
- * resident until the last context using them is destroyed. */
+ * resident until the last context using them is destroyed.  Out-of-range
+ * options fail with SWE_HIP_EINVAL before any device call. */
 #define SWE_HIP_KERNEL_AUTO 0   /* single state: PAIR for N >= 100000 with a
                                    31-point stencil, else QUADW              */
 #define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
@@ -105,12 +107,15 @@ typedef struct swe_hip_options {
     int member_block;
     swe_hip_ctx *share;
     int kernel;                 /* SWE_HIP_KERNEL_* (members == 1 only)      */
-    int prefetch;               /* stencil slots in flight per wave: 0
-                                   (compiler schedule), 4 or 8              */
+    int prefetch;               /* stencil-loop schedule: 0 (compiler's),
+                                   4 or 8 (ring of slots; member-pair kernel:
+                                   ring depth 2 or 4, default 3)            */
     int node_order;             /* SWE_HIP_ORDER_* of the device copy (ignored
                                    with share: the mesh's order is used)    */
     int tiled;                  /* 1: LDS-staged ensemble kernel (members
-                                   >= 2, member_block 4, 8 or 16)           */
+                                   >= 2, member_block 4, 8 or 16: one member
+                                   per lane; member_block 32 with a 31-point
+                                   stencil: persistent member-lane tiles)   */
     int slot_block;             /* stencil arrays: 0 slot-major [nnbr][n];
                                    16 node blocks [n/16][nnbr][16] (ignored
                                    with share)                              */
