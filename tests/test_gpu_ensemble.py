@@ -121,6 +121,25 @@ def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb):
             assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 4)), m
 
 
+def test_member_lane_tiles_host_order_substeps_exact(pkg, pyoracle):
+    """Member-lane LDS tiles on the caller's (RCM) node order: the drop-in
+    substep path (RHS kernel + separate axpys) and fused steps both give the
+    oracle's bits."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
+    print("case: member-lane tiles B=32 host order, substeps + fused, exact", flush=True)
+    H0 = _members(pkg, prob, 32)
+    orc = pyoracle.Oracle(prob)
+    with pkg.runtime.HipContext(prob, members=32, member_block=32, exact=True,
+                                tiled=True) as ctx:
+        for m in range(32):
+            ctx.set_state(H0[m], m)
+        for s in range(4):
+            ctx.rk_substep(s)
+        ctx.rk_steps(2)
+        for m in (0, 13, 31):
+            assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 3)), m
+
+
 @pytest.mark.parametrize("mb", [16, 32])
 def test_tiled_fast_composition_and_active(pkg, mb):
     prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl")
