@@ -122,7 +122,7 @@ def dominant_kernel(args):
     """Name of the stage kernel the options select (the one timed)."""
     if args.members == 1:
         if args.kernel == "auto":
-            return "k_stage_pair" if args.nodes >= 100000 else "k_stage_quadw"
+            return "k_stage_pair"
         return {"quadw": "k_stage_quadw", "quad": "k_stage_quad",
                 "lane": "k_stage_lane", "pair": "k_stage_pair"}[args.kernel]
     if args.tiled:

@@ -88,8 +88,8 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
  * the static data is reused (s may then be NULL): the static data stay
  * resident until the last context using them is destroyed.  Out-of-range
  * options fail with SWE_HIP_EINVAL before any device call. */
-#define SWE_HIP_KERNEL_AUTO 0   /* single state: PAIR for N >= 100000 with a
-                                   31-point stencil, else QUADW              */
+#define SWE_HIP_KERNEL_AUTO 0   /* single state: PAIR with a 31-point
+                                   stencil, else QUADW                       */
 #define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
 #define SWE_HIP_KERNEL_LANE 2   /* single state: 1 lane per node             */
 #define SWE_HIP_KERNEL_QUADW 3  /* single state: 4 lanes per node, weights,

@@ -7,8 +7,8 @@ namespace {
 template <int STAGE, bool EX, int NN>
 void launch_quad(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
-    st.nblk = blocks_x(c, 4L * c->n);
-    const dim3 g(st.nblk), b(kBlock);
+    st.nblk = blocks_x(c, 4L * c->n, kSingleBlock);
+    const dim3 g(st.nblk), b(kSingleBlock);
     if (NN > 0 && c->prefetch == 8)
         k_stage_quad<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
     else if (NN > 0 && c->prefetch == 4)
@@ -20,8 +20,8 @@ void launch_quad(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 template <int STAGE, bool EX, int NN>
 void launch_lane(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
-    st.nblk = blocks_x(c, (long)c->n);
-    const dim3 g(st.nblk), b(kBlock);
+    st.nblk = blocks_x(c, (long)c->n, kSingleBlock);
+    const dim3 g(st.nblk), b(kSingleBlock);
     if (NN > 0 && (EX || c->prefetch == 4))
         k_stage_lane<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
     else if (NN > 0 && c->prefetch == 8)
@@ -33,8 +33,8 @@ void launch_lane(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 template <int STAGE, bool EX, int NN>
 void launch_quadw(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
-    st.nblk = blocks_x(c, 4L * c->n);
-    const dim3 g(st.nblk), b(kBlock);
+    st.nblk = blocks_x(c, 4L * c->n, kSingleBlock);
+    const dim3 g(st.nblk), b(kSingleBlock);
     if (NN > 0 && c->prefetch == 8)
         k_stage_quadw<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
     else if (NN > 0 && c->prefetch == 4)
@@ -47,8 +47,8 @@ template <int STAGE, bool EX, int NN>
 void launch_pair(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
     if constexpr (NN > 0) {
-        st.nblk = blocks_x(c, 2L * c->n);
-        k_stage_pair<STAGE, EX, NN><<<dim3(st.nblk), dim3(kBlock), 0, c->stream>>>(m, st);
+        st.nblk = blocks_x(c, 2L * c->n, kSingleBlock);
+        k_stage_pair<STAGE, EX, NN><<<dim3(st.nblk), dim3(kSingleBlock), 0, c->stream>>>(m, st);
     } else {
         launch_quadw<STAGE, EX, NN>(c, m, st);   // needs a compile-time stencil width
     }

@@ -12,6 +12,11 @@
 namespace swe {
 
 constexpr int kBlock = 256;            // threads per workgroup, stage kernels
+// Threads per workgroup of the single-state stage kernels: one wave, so a
+// launch's workgroups spread over the 256 CUs within one wave of each other
+// (at N = 40962 the pair kernel has 1281 waves: 256-thread workgroups put
+// two on 65 CUs and one on the rest).
+constexpr int kSingleBlock = 64;
 constexpr int kTiledBlock = 512;       // threads per workgroup, tiled kernel
 constexpr int kTiledLds = 80 * 1024;   // LDS per tiled workgroup: two per CU
 
@@ -94,9 +99,9 @@ inline bool exact(const swe_hip_ctx *c) { return (c->flags & SWE_HIP_EXACT) != 0
 
 inline long state_count(const swe_hip_ctx *c) { return 4L * c->n * c->B; }
 
-inline int blocks_x(const swe_hip_ctx *c, long threads)
+inline int blocks_x(const swe_hip_ctx *c, long threads, int block = kBlock)
 {
-    long nb = (threads + kBlock - 1) / kBlock;
+    long nb = (threads + block - 1) / block;
     nb = (nb + 7) / 8 * 8;   // multiple of 8 for the XCD remap
     return (int)nb;
 }

@@ -561,12 +561,11 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->d_geo = m->d_geo;
     c->B = o->members;
     c->MB = mb;
-    // auto: the pair mapping halves the address-unit work per node and wins
-    // once the mesh fills the chip (C3: +5 %); quadw has twice the waves
-    // for smaller meshes.
+    // auto: the pair mapping (half the address-unit work per node of the
+    // quad mapping) with one-wave workgroups is the fastest measured at
+    // N = 40962, 163842 and 655362; it needs a compile-time stencil width.
     if (o->kernel == SWE_HIP_KERNEL_AUTO)
-        c->kernel = (m->n >= 100000 && m->nnbr == 31) ? SWE_HIP_KERNEL_PAIR
-                                                      : SWE_HIP_KERNEL_QUADW;
+        c->kernel = m->nnbr == 31 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_QUADW;
     else
         c->kernel = o->kernel;
     c->prefetch = o->prefetch;

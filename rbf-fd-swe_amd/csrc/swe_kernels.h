@@ -325,7 +325,7 @@ struct QuadSlot {
 template <int STAGE, bool EXACT, int NNBR, int PF>
 __global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
 {
-    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
     const int node_raw = t >> 2;
     const int k = t & 3;
     const bool live = node_raw < m.n;
@@ -388,7 +388,7 @@ struct QuadWSlot {
 template <int STAGE, bool EXACT, int NNBR, int PF>
 __global__ __launch_bounds__(256) void k_stage_quadw(DevMesh m, DevStage st)
 {
-    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
     const int node_raw = t >> 2;
     const int k = t & 3;
     const bool live = node_raw < m.n;
@@ -490,7 +490,7 @@ template <int STAGE, bool EXACT, int NNBR>
 __global__ __launch_bounds__(256) void k_stage_pair(DevMesh m, DevStage st)
 {
     static_assert(NNBR > 0, "compile-time stencil width");
-    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
     const int node_raw = t >> 1;
     const int k = t & 1;
     const bool live = node_raw < m.n;
@@ -573,7 +573,7 @@ struct LaneSlot {
 template <int STAGE, bool EXACT, int NNBR, int PF>
 __global__ __launch_bounds__(256) void k_stage_lane(DevMesh m, DevStage st)
 {
-    const int t = xcd_block(st.nblk) * 256 + (int)threadIdx.x;
+    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
     const bool live = t < m.n;
     const int node = live ? t : m.n - 1;
     const int nn = NNBR > 0 ? NNBR : m.nnbr;
