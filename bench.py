@@ -126,7 +126,7 @@ def dominant_kernel(args):
         return {"quadw": "k_stage_quadw", "quad": "k_stage_quad",
                 "lane": "k_stage_lane", "pair": "k_stage_pair"}[args.kernel]
     if args.tiled:
-        return "k_stage_tmem" if args.member_block == 32 else "k_stage_tiled"
+        return "k_stage_tspec" if args.member_block == 32 else "k_stage_tiled"
     return "k_stage_mpair" if args.member_block in (0, 64) and args.members % 64 == 0 \
         else "k_stage_members"
 
@@ -157,7 +157,8 @@ def ensemble_leg(pkg, torch, steps=500):
     base = pkg.runtime.HipContext(prob, order="morton")
     candidates = [("member pairs (16-byte gathers), pipeline depth 3",
                    dict(member_block=64, prefetch=0)),
-                  ("LDS tiles with member lanes, member block 32, persistent",
+                  ("LDS tiles with member lanes, member block 32, loader and compute waves "
+                   "over two LDS images, persistent",
                    dict(member_block=32, tiled=True))]
     best = None
     for name, kw in candidates:

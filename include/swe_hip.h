@@ -115,7 +115,9 @@ typedef struct swe_hip_options {
     int tiled;                  /* 1: LDS-staged ensemble kernel (members
                                    >= 2, member_block 4, 8 or 16: one member
                                    per lane; member_block 32 with a 31-point
-                                   stencil: persistent member-lane tiles)   */
+                                   stencil: persistent member-lane tiles,
+                                   loader and compute waves over two LDS
+                                   images, 160 KB per workgroup)            */
     int slot_block;             /* stencil arrays: 0 slot-major [nnbr][n];
                                    16 node blocks [n/16][nnbr][16] (ignored
                                    with share)                              */

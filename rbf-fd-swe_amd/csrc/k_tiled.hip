@@ -27,19 +27,18 @@ void launch_tiled(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
     k_stage_tiled<STAGE, EX, MBv, NN><<<dim3(st.nblk), dim3(kTiledBlock), lds, c->stream>>>(m, tl, st);
 }
 
-// Member lanes over LDS tiles (member block 32, 31-point stencils).
+// Member lanes over LDS tiles (member block 32, 31-point stencils): loader
+// and compute waves over two LDS images, one persistent workgroup of 8 waves
+// per CU.
 template <int STAGE, bool EX>
-void launch_tmem(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+void launch_tspec(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {
     const DevTiles tl = tiles_of(c, 32);
-    // persistent: two workgroups per CU (LDS-bound), each over a contiguous
-    // chunk of (tile, member block) items
     const long items = (long)tl.ntiles * tl.nmb;
-    const long wgs = 2L * c->ncu;
+    const long wgs = c->ncu;
     st.nblk = (int)(((items < wgs ? items : wgs) + 7) / 8 * 8);
-    const size_t lds =
-        sizeof(double) * (kTmemWtsDoubles + kTmemGeoDoubles + 4 * 32 * (size_t)c->tiles.umax);
-    k_stage_tmem<STAGE, EX, 31><<<dim3(st.nblk), dim3(256), lds, c->stream>>>(m, tl, st);
+    const size_t lds = sizeof(double) * 2 * (size_t)kTmemItemDoubles;
+    k_stage_tspec<STAGE, EX, 31><<<dim3(st.nblk), dim3(512), lds, c->stream>>>(m, tl, st);
 }
 
 }  // namespace
@@ -63,9 +62,9 @@ void by_flags(const swe_hip_ctx *c, const DevStage &st)
     if (c->MB == 32) {   // set up only for 31-point stencils
         const DevMesh m = mesh_of(c);
         if (exact(c))
-            launch_tmem<STAGE, true>(c, m, st);
+            launch_tspec<STAGE, true>(c, m, st);
         else
-            launch_tmem<STAGE, false>(c, m, st);
+            launch_tspec<STAGE, false>(c, m, st);
         return;
     }
     const bool nn31 = c->nnbr == 31;

@@ -512,6 +512,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
     if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_PAIR)
         return fail(SWE_HIP_EINVAL, "kernel must be one of SWE_HIP_KERNEL_*");
+    if (o->tiled < 0 || o->tiled > 1)
+        return fail(SWE_HIP_EINVAL, "tiled must be 0 or 1");
     if (o->prefetch != 0 && o->prefetch != 4 && o->prefetch != 8)
         return fail(SWE_HIP_EINVAL, "prefetch must be 0, 4 or 8");
     if (o->node_order != SWE_HIP_ORDER_HOST && o->node_order != SWE_HIP_ORDER_MORTON)
@@ -593,10 +595,16 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         rc = check_launch("k_set_int");
     }
     if (!rc && o->tiled) {
+        int lds_max = 0;
         if (c->MB == 32 && c->B >= 32) {
             if (c->nnbr != 31)
                 rc = fail(SWE_HIP_EINVAL, "the member-lane tiled kernel (member block 32) needs "
                                           "a 31-point stencil");
+            else if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                           device) != hipSuccess ||
+                     (size_t)lds_max < sizeof(double) * 2 * (size_t)kTmemItemDoubles)
+                rc = fail(SWE_HIP_EDEVICE, "the member-lane tiled kernel needs 160 KB of LDS "
+                                           "per workgroup");
             else
                 rc = build_tiles(c, kTmemTile, kTmemRows);
         } else if (c->B < 2 || (c->MB != 4 && c->MB != 8 && c->MB != 16)) {

@@ -276,7 +276,7 @@ __device__ __forceinline__ Geo load_geo_quad(const double *__restrict__ geo, int
 // iterations.  The barriers carry no data dependency, so they shape the
 // compiler's schedule without fixing it (in exact mode they keep it from
 // precomputing all 31x16 products ahead of the serial sums); the kernels
-// that need a firm rolling window (k_stage_mpair, k_stage_tmem) instead tie
+// that need a firm rolling window (k_stage_mpair, k_stage_tspec) instead tie
 // each next address to the running sums with an empty asm.  Accumulation
 // is sequential in j for every sum in all forms.
 template <int NNBR, int PF, typename LoadFn, typename AccFn, typename Slot>
@@ -948,24 +948,23 @@ __device__ __forceinline__ Geo load_geo_pair(const double *__restrict__ geo, int
 }
 
 // ---------------------------------------------------------------------------
-// Ensemble, LDS tiles with member lanes (member block 32).  A workgroup
-// takes one tile (<= kTmemTile consecutive Morton nodes whose stencils touch
+// Ensemble, LDS tiles with member lanes (member block 32).  An item is one
+// tile (<= kTmemTile consecutive Morton nodes whose stencils touch
 // <= kTmemRows distinct nodes) for 32 members.  A wave64 global load costs
-// the CU's address unit 16 cycles whatever its width, an LDS b128 read ~5,
-// so the kernel moves every byte it can through LDS:
+// the CU's address unit 16 cycles whatever its width, an LDS read ~5, so
+// the kernels move every byte they can through LDS:
 //  * the tile's neighbour rows are copied into LDS once -- 16-byte global
 //    loads, one row (4 variables x 32 members, 1 KB) per wave instruction --
 //    instead of being gathered from L2 once per stencil that names them;
-//  * each wave stages the weights (31 x 32 B) and geometry of its two nodes
-//    in a wave-private LDS slice;
+//    LDS rows keep the global layout [var][member] (16-byte writes, 8-byte
+//    reads, contiguous across lanes);
+//  * the weights (31 x 32 B), geometry and 16-bit stencil positions of each
+//    wave's two nodes are staged in a wave-private LDS slice;
 //  * lane = member, two nodes per wave: per slot the lane reads its member's
-//    (u, v, w, h) (two b128 reads) and its half-wave's node weights (two b128
-//    broadcasts); stencil positions (16-bit, node-major, 64 B per node) come
-//    through one scalar load per node.
-// All global loads of an item (rows, weights, geometry, the epilogue's own
-// state, H and D) are issued in one burst, one item ahead of its compute.  LDS row layout [row][var pair][member][2 vars]
-// keeps reads contiguous across lanes.  Per member the sums, the RHS and
-// the stage update are those of k_stage_members (bitwise in exact mode).
+//    (u, v, w, h) (two ds_read2_b64) and its node's four weights (two b128
+//    broadcasts, or none with the DPP weight path of tmem_compute).
+// Per member the sums, the RHS and the stage update are those of
+// k_stage_members (bitwise in exact mode).
 constexpr int kTmemTile = 8;    // nodes per tile: 4 waves x 2 nodes
 constexpr int kTmemRows = 70;   // distinct rows per tile: 70 KB of LDS (+10 KB: 2 workgroups fill 160 KB)
 constexpr int kTmemWtsDoubles = 4 * 2 * 32 * 4;   // weights [wave][node][slot][4]
@@ -990,204 +989,279 @@ __device__ __forceinline__ void stage_update_pre(const DevStage &st, long e, dou
     }
 }
 
-// Global loads of one (tile, member block) item, held in registers until
-// they are written to LDS (fill rows, weights, geometry) or used by the
-// epilogue (own state, H, D).
-struct TmemLoads {
+// One (tile, member block) item as wave slice `wid` (0..3) sees it: the
+// wave owns tile nodes 2 wid, 2 wid + 1 (nA, nB) and copies tile rows
+// [wid * kTmemFill, (wid + 1) * kTmemFill).  All fields wave-uniform.
+struct TmemItem {
+    int tile, t0, tcount, ucount, col0, nA, nB;
+    bool live, hasB;
+};
+
+__device__ __forceinline__ TmemItem tmem_item(const DevTiles &tl, long it, int wid,
+                                              int active_members)
+{
+    TmemItem I;
+    const int tile = (int)(it / tl.nmb);
+    const int mb = (int)(it - (long)tile * tl.nmb);
+    I.tile = tile;
+    I.t0 = c_i32(tl.tile_start + tile);
+    I.tcount = c_i32(tl.tile_start + tile + 1) - I.t0;
+    I.ucount = c_i32(tl.ustart + tile + 1) - c_i32(tl.ustart + tile);
+    I.col0 = mb * 32;
+    const int i0 = 2 * wid;
+    I.live = i0 < I.tcount && I.col0 < active_members;
+    I.hasB = i0 + 1 < I.tcount;
+    I.nA = I.t0 + (i0 < I.tcount ? i0 : 0);
+    I.nB = I.hasB ? I.nA + 1 : I.nA;
+    return I;
+}
+
+// LDS image of one item (doubles): weights [4][2][32][4], geometry
+// [4][2][32], rows [kTmemRows][2][32][2].
+constexpr int kTmemItemDoubles = kTmemWtsDoubles + kTmemGeoDoubles + 128 * kTmemRows;
+
+// Global loads that fill an item's LDS image (wave slice wid's share),
+// held in registers until written.  gx is lane (half, k)'s 8-byte share of
+// its node's geometry row: field k for k < 17, stencil positions
+// 4 (k - 24) .. 4 (k - 24) + 3 for k >= 24 (filler for 17..23).
+struct TmemFill {
     double2 fx[kTmemFill];
     double2 xa, xb;
     double gx;
+};
+
+// Geometry-row slot of the 32 16-bit stencil positions (8 doubles).
+constexpr int kTmemPosSlot = 24;
+static_assert(GEO_N <= kTmemPosSlot, "geometry fields and positions share a 32-double row");
+
+template <int NNBR>
+__device__ __forceinline__ TmemFill tmem_fill_load(const DevMesh &m, const DevTiles &tl,
+                                                   const double *__restrict__ src, long B,
+                                                   const TmemItem &I, int wid, int lane)
+{
+    const long ld = slot_stride(m);
+    const int jw = (lane >> 1) < NNBR ? (lane >> 1) : 0;
+    const int k = lane & 31;
+    const int fv = lane >> 4;
+    const int fp = lane & 15;
+    TmemFill L;
+    L.xa = *reinterpret_cast<const double2 *>(m.wts + 4L * slot_base<NNBR>(m, I.nA) +
+                                              4L * jw * ld + 2 * (lane & 1));
+    L.xb = *reinterpret_cast<const double2 *>(m.wts + 4L * slot_base<NNBR>(m, I.nB) +
+                                              4L * jw * ld + 2 * (lane & 1));
+    const int node = (lane >> 5) ? I.nB : I.nA;
+    const double *gsrc =
+        k >= kTmemPosSlot
+            ? reinterpret_cast<const double *>(tl.lidx_node + 32L * node) + (k - kTmemPosSlot)
+            : m.geo + (long)(k < GEO_N ? k : 0) * m.ld + node;
+    L.gx = *gsrc;
+    const double *__restrict__ s0 = src + I.col0 + 2 * fp + (long)fv * B;
+    // one merged scalar load of row ids at a fixed per-tile stride (no wait
+    // for the tile's offset); the zero padding past the tile's count is a
+    // readable node id, loaded but not stored
+    const int *ul = tl.ulist_tile + (long)I.tile * kTmemStride + wid * kTmemFill;
+#pragma unroll
+    for (int q = 0; q < kTmemFill; ++q)
+        L.fx[q] = *reinterpret_cast<const double2 *>(s0 + 4L * c_i32(ul + q) * B);
+    return L;
+}
+
+__device__ __forceinline__ void tmem_fill_store(double *img, const TmemItem &I,
+                                                const TmemFill &L, int wid, int lane)
+{
+    double *wl = img;
+    double *gl = img + kTmemWtsDoubles;
+    double *rows = gl + kTmemGeoDoubles;
+    double2 *dst = reinterpret_cast<double2 *>(wl) + wid * 128 + lane;
+    dst[0] = L.xa;    // [wid][0][slot][..]
+    dst[64] = L.xb;   // [wid][1][slot][..]
+    gl[wid * 64 + lane] = L.gx;   // [wid][half][geometry 0..16 | positions 24..31]
+    // rows [row][var][32 members] as in global memory: lane (var fv, fp)
+    // writes members 2 fp, 2 fp + 1 of variable fv, 16 contiguous bytes
+    double2 *d0 = reinterpret_cast<double2 *>(rows) + lane;
+#pragma unroll
+    for (int q = 0; q < kTmemFill; ++q) {
+        const int rr = wid * kTmemFill + q;
+        if (rr < I.ucount)   // a fresh double2 keeps fx in registers (one b128 write)
+            d0[rr * 64] = make_double2(L.fx[q].x, L.fx[q].y);
+    }
+}
+
+// The epilogue's global operands of a lane's (node, member): own state, H, D.
+struct TmemOwn {
     double ou, ov, ow, oh;
     double hv[4], dv[4];
 };
 
-// Persistent over (tile, member block) items (member blocks of a tile
-// adjacent, tiles in Morton order).  The global loads of a workgroup's next
-// item are issued before its current item is computed from LDS, so their
-// latency hides behind the compute instead of stalling the CU.  No workgroup
-// waits for another: each runs its share of items and exits.
-template <int STAGE, bool EXACT, int NNBR>
-__global__ __launch_bounds__(256) void k_stage_tmem(DevMesh m, DevTiles tl, DevStage st)
+template <int STAGE>
+__device__ __forceinline__ TmemOwn tmem_own_load(const DevStage &st, const double *__restrict__ src,
+                                                 long B, const TmemItem &I, int lane)
 {
-    static_assert(NNBR > 0 && NNBR <= 32, "node-major positions hold 32 slots");
-    constexpr int MB = 32;
-    // weights [4][2][32][4], geometry [4][2][32], rows [ucount][2][32][2]
-    extern __shared__ double lds[];
-    double *wl = lds;
-    double *gl = lds + kTmemWtsDoubles;
-    double *rows = gl + kTmemGeoDoubles;
-    // XCD x (hardware blocks are dealt round-robin over the 8 XCDs) owns
-    // items [x R, (x+1) R); its W workgroups interleave over them (j, j + W,
-    // ...), so the items in flight on one XCD are adjacent tiles that share
-    // halo rows and weights in that XCD's L2.
+    TmemOwn O;
+    const int node = (lane >> 5) ? I.nB : I.nA;
+    const long e = 4L * node * B + I.col0 + (lane & 31);
+    O.ou = src[e]; O.ov = src[e + B]; O.ow = src[e + 2 * B]; O.oh = src[e + 3 * B];
+#pragma unroll
+    for (int var = 0; var < 4; ++var) {
+        O.hv[var] = STAGE != ST_RHS ? st.h[e + var * B] : 0.0;
+        O.dv[var] = (STAGE != ST_RHS && STAGE != ST_0) ? st.d[e + var * B] : 0.0;
+    }
+    return O;
+}
+
+// Stencil sums of wave slice wid's two nodes from an item's LDS image, then
+// the pointwise RHS and the stage update of every (node, member) lane.
+template <int STAGE, bool EXACT, int NNBR, int DEPTH>
+__device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &tl,
+                                             const DevStage &st, long B, const double *img,
+                                             const TmemItem &I, const TmemOwn &O, int wid,
+                                             int lane)
+{
+    const int ml = lane & 31;
+    const int half = lane >> 5;
+    const double *wl = img;
+    const double *gl = img + kTmemWtsDoubles;
+    const double *rows = gl + kTmemGeoDoubles;
+    // the half-wave's node positions (16-bit, 2 per dword), broadcast reads
+    const double *gp = gl + wid * 64 + half * 32;
+    unsigned p[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(gp + kTmemPosSlot + 2 * q);
+        p[4 * q] = v.x; p[4 * q + 1] = v.y; p[4 * q + 2] = v.z; p[4 * q + 3] = v.w;
+    }
+    auto row_off = [&](int j) {
+        const unsigned a = p[j >> 1];
+        return (int)((j & 1) ? (a >> 16) : (a & 0xffffu)) * 128;
+    };
+    const double *lrow = rows + ml;
+    const double2 *wrow = reinterpret_cast<const double2 *>(wl) + wid * 128 + half * 64;
+    // The LDS reads of a slot (member row, weights) run DEPTH slots ahead,
+    // in issue order, so waiting for slot j leaves the later slots in
+    // flight; slot j + DEPTH's reads are tied to the sums after slot j
+    // (bounded live ranges in exact mode too).
+    double2 ruv[DEPTH], rwh[DEPTH], rw01[DEPTH], rw23[DEPTH];
+    auto issue = [&](int j, int off) {
+        ruv[j % DEPTH] = make_double2(lrow[off], lrow[off + 32]);        // u, v
+        rwh[j % DEPTH] = make_double2(lrow[off + 64], lrow[off + 96]);   // w, h
+        rw01[j % DEPTH] = wrow[2 * j];
+        rw23[j % DEPTH] = wrow[2 * j + 1];
+    };
+    Sums S = {};
+    static_for<DEPTH>(
+        [&](auto jc) { issue(decltype(jc)::value, row_off(decltype(jc)::value)); });
+    static_for<NNBR>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const double2 uv = ruv[j % DEPTH], wh = rwh[j % DEPTH];
+        acc_slot<EXACT>(S, rw01[j % DEPTH], rw23[j % DEPTH], uv.x, uv.y, wh.x, wh.y);
+        if constexpr (j + DEPTH < NNBR) {
+            int off = row_off(j + DEPTH);
+            asm volatile(""
+                         : "+v"(off)
+                         : "v"(S.dudx), "v"(S.dvdx), "v"(S.dwdx), "v"(S.dhdx),
+                           "v"(S.dudy), "v"(S.dvdy), "v"(S.dwdy), "v"(S.dhdy),
+                           "v"(S.dudz), "v"(S.dvdz), "v"(S.dwdz), "v"(S.dhdz),
+                           "v"(S.lu), "v"(S.lv), "v"(S.lw), "v"(S.lh));
+            issue(j + DEPTH, off);
+        }
+    });
+    Geo g;
+    g.x = gp[G_X]; g.y = gp[G_Y]; g.z = gp[G_Z]; g.f = gp[G_F]; g.ghm = gp[G_GHM];
+    g.pu0 = gp[G_PU]; g.pu1 = gp[G_PU + 1]; g.pu2 = gp[G_PU + 2];
+    g.pv0 = gp[G_PV]; g.pv1 = gp[G_PV + 1]; g.pv2 = gp[G_PV + 2];
+    g.pw0 = gp[G_PW]; g.pw1 = gp[G_PW + 1]; g.pw2 = gp[G_PW + 2];
+    g.gx = gp[G_GG]; g.gy = gp[G_GG + 1]; g.gz = gp[G_GG + 2];
+    double F[4];
+    rhs_point(S, O.ou, O.ov, O.ow, O.oh, g, m.gh0, F);
+    if (!half || I.hasB) {
+        const int node = half ? I.nB : I.nA;
+        const long e = 4L * node * B + I.col0 + ml;
+#pragma unroll
+        for (int var = 0; var < 4; ++var)
+            stage_update_pre<STAGE, EXACT>(st, e + var * B, F[var], O.hv[var], O.dv[var]);
+    }
+}
+
+// Items of a persistent launch: XCD x (hardware blocks are dealt
+// round-robin over the 8 XCDs) owns items [x R, (x+1) R); its W workgroups
+// interleave over them (j, j + W, ...), so the items in flight on one XCD
+// are adjacent tiles that share halo rows and weights in that XCD's L2.
+struct TmemRange {
+    long it0, it1, W;
+};
+
+__device__ __forceinline__ TmemRange tmem_range(const DevTiles &tl, int nblk)
+{
     const long items = (long)tl.ntiles * tl.nmb;
     const long R = (items + 7) / 8;
     const int xcd = (int)blockIdx.x & 7;
-    const long W = st.nblk >> 3;
-    const long it0 = (long)xcd * R + ((long)blockIdx.x >> 3);
+    TmemRange r;
+    r.W = nblk >> 3;
+    r.it0 = (long)xcd * R + ((long)blockIdx.x >> 3);
     const long xend = (long)(xcd + 1) * R;
-    const long it1 = xend < items ? xend : items;
+    r.it1 = xend < items ? xend : items;
+    return r;
+}
+
+// The same items with specialised waves and two LDS images (160 KB, one
+// workgroup per CU): waves 4..7 fill one image while waves 0..3 compute
+// from the other, one barrier per item.  Loader wave 4 + w fills the share
+// of compute wave w and keeps two items in flight: it issues the loads of
+// item i + 2 before writing item i + 1 (loaded one barrier earlier) into
+// the free image.  A compute wave issues its own epilogue loads (own
+// state, H, D) before its stencil loop.
+template <int STAGE, bool EXACT, int NNBR>
+__global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, DevStage st)
+{
+    static_assert(NNBR > 0 && NNBR <= 32, "node-major positions hold 32 slots");
+    extern __shared__ double lds[];   // two item images
+    const TmemRange r = tmem_range(tl, st.nblk);
     const long B = st.members;
     const int lane = (int)threadIdx.x & 63;
-    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const long ld = slot_stride(m);
-    const double *__restrict__ src = st.src;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int wid = wv & 3;
     const int active_members = c_i32(st.active);
-    const int ml = lane & 31;
-    const int half = lane >> 5;
-    const int fv = lane >> 4;
-    const int fp = lane & 15;
-    const int jw = (lane >> 1) < NNBR ? (lane >> 1) : 0;
-    const int gf = ml < GEO_N ? ml : 0;
-
-    // Item geometry (wave-uniform).
-    struct Item {
-        int tile, t0, tcount, ucount, col0, nA, nB;
-        bool live, hasB;
-    };
-    auto item_of = [&](long it) {
-        Item I;
-        const int tile = (int)(it / tl.nmb);
-        const int mb = (int)(it - (long)tile * tl.nmb);
-        I.tile = tile;
-        I.t0 = c_i32(tl.tile_start + tile);
-        I.tcount = c_i32(tl.tile_start + tile + 1) - I.t0;
-        I.ucount = c_i32(tl.ustart + tile + 1) - c_i32(tl.ustart + tile);
-        I.col0 = mb * MB;
-        const int i0 = 2 * wid;
-        I.live = i0 < I.tcount && I.col0 < active_members;
-        I.hasB = i0 + 1 < I.tcount;
-        I.nA = I.t0 + (i0 < I.tcount ? i0 : 0);
-        I.nB = I.hasB ? I.nA + 1 : I.nA;
-        return I;
-    };
-    auto load = [&](const Item &I) {
-        TmemLoads L;
-        L.xa = *reinterpret_cast<const double2 *>(
-            m.wts + 4L * slot_base<NNBR>(m, I.nA) + 4L * jw * ld + 2 * (lane & 1));
-        L.xb = *reinterpret_cast<const double2 *>(
-            m.wts + 4L * slot_base<NNBR>(m, I.nB) + 4L * jw * ld + 2 * (lane & 1));
-        const int node = half ? I.nB : I.nA;
-        L.gx = m.geo[(long)gf * m.ld + node];
-        const double *__restrict__ s0 = src + I.col0 + 2 * fp + (long)fv * B;
-        // wave w copies rows [w * kTmemFill, (w + 1) * kTmemFill) of the
-        // tile: one merged scalar load of ids at a fixed per-tile stride (no
-        // wait for the tile's offset); the zero padding past the tile's
-        // count is a readable node id, loaded but not stored
-        const int *ul = tl.ulist_tile + (long)I.tile * kTmemStride + wid * kTmemFill;
-#pragma unroll
-        for (int k = 0; k < kTmemFill; ++k)
-            L.fx[k] = *reinterpret_cast<const double2 *>(s0 + 4L * c_i32(ul + k) * B);
-        const long e = 4L * node * B + I.col0 + ml;
-        L.ou = src[e]; L.ov = src[e + B]; L.ow = src[e + 2 * B]; L.oh = src[e + 3 * B];
-#pragma unroll
-        for (int var = 0; var < 4; ++var) {
-            L.hv[var] = STAGE != ST_RHS ? st.h[e + var * B] : 0.0;
-            L.dv[var] = (STAGE != ST_RHS && STAGE != ST_0) ? st.d[e + var * B] : 0.0;
-        }
-        return L;
-    };
-    auto to_lds = [&](const Item &I, const TmemLoads &L) {
-        double2 *dst = reinterpret_cast<double2 *>(wl) + wid * 128 + lane;
-        dst[0] = L.xa;    // [wid][0][slot][..]
-        dst[64] = L.xb;   // [wid][1][slot][..]
-        gl[wid * 64 + half * 32 + ml] = L.gx;
-        double *d0 = rows + (fv >> 1) * 64 + 4 * fp + (fv & 1);
-#pragma unroll
-        for (int k = 0; k < kTmemFill; ++k) {
-            const int rr = wid * kTmemFill + k;
-            if (rr < I.ucount) {
-                d0[rr * 128] = L.fx[k].x;
-                d0[rr * 128 + 2] = L.fx[k].y;
-            }
-        }
-    };
-    auto compute = [&](const Item &I, const TmemLoads &L) {
-        unsigned pa[16], pb[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            pa[k] = s_u32(reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * I.nA) + k);
-            pb[k] = s_u32(reinterpret_cast<const unsigned *>(tl.lidx_node + 32L * I.nB) + k);
-        }
-        auto row_off = [&](int j) {
-            const unsigned a = pa[j >> 1], b = pb[j >> 1];
-            const int ra = (int)((j & 1) ? (a >> 16) : (a & 0xffffu));
-            const int rb = (int)((j & 1) ? (b >> 16) : (b & 0xffffu));
-            return (half ? rb : ra) * 128;
-        };
-        const double *lrow = rows + 2 * ml;
-        const double2 *wrow = reinterpret_cast<const double2 *>(wl) + wid * 128 + half * 64;
-        // All four LDS reads of a slot (member row, weights) run DEPTH slots
-        // ahead, in issue order, so waiting for slot j leaves the later
-        // slots in flight; slot j + DEPTH's reads are tied to the sums after
-        // slot j (bounded live ranges in exact mode too).
-        constexpr int DEPTH = 3;
-        double2 ruv[DEPTH], rwh[DEPTH], rw01[DEPTH], rw23[DEPTH];
-        auto issue = [&](int j, int off) {
-            ruv[j % DEPTH] = *reinterpret_cast<const double2 *>(lrow + off);
-            rwh[j % DEPTH] = *reinterpret_cast<const double2 *>(lrow + off + 64);
-            rw01[j % DEPTH] = wrow[2 * j];
-            rw23[j % DEPTH] = wrow[2 * j + 1];
-        };
-        Sums S = {};
-        static_for<DEPTH>(
-            [&](auto jc) { issue(decltype(jc)::value, row_off(decltype(jc)::value)); });
-        static_for<NNBR>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            const double2 uv = ruv[j % DEPTH], wh = rwh[j % DEPTH];
-            acc_slot<EXACT>(S, rw01[j % DEPTH], rw23[j % DEPTH], uv.x, uv.y, wh.x, wh.y);
-            if constexpr (j + DEPTH < NNBR) {
-                int off = row_off(j + DEPTH);
-                asm volatile(""
-                             : "+v"(off)
-                             : "v"(S.dudx), "v"(S.dvdx), "v"(S.dwdx), "v"(S.dhdx),
-                               "v"(S.dudy), "v"(S.dvdy), "v"(S.dwdy), "v"(S.dhdy),
-                               "v"(S.dudz), "v"(S.dvdz), "v"(S.dwdz), "v"(S.dhdz),
-                               "v"(S.lu), "v"(S.lv), "v"(S.lw), "v"(S.lh));
-                issue(j + DEPTH, off);
-            }
-        });
-        const double *gp = gl + wid * 64 + half * 32;
-        Geo g;
-        g.x = gp[G_X]; g.y = gp[G_Y]; g.z = gp[G_Z]; g.f = gp[G_F]; g.ghm = gp[G_GHM];
-        g.pu0 = gp[G_PU]; g.pu1 = gp[G_PU + 1]; g.pu2 = gp[G_PU + 2];
-        g.pv0 = gp[G_PV]; g.pv1 = gp[G_PV + 1]; g.pv2 = gp[G_PV + 2];
-        g.pw0 = gp[G_PW]; g.pw1 = gp[G_PW + 1]; g.pw2 = gp[G_PW + 2];
-        g.gx = gp[G_GG]; g.gy = gp[G_GG + 1]; g.gz = gp[G_GG + 2];
-        double F[4];
-        rhs_point(S, L.ou, L.ov, L.ow, L.oh, g, m.gh0, F);
-        if (!half || I.hasB) {
-            const int node = half ? I.nB : I.nA;
-            const long e = 4L * node * B + I.col0 + ml;
-#pragma unroll
-            for (int var = 0; var < 4; ++var)
-                stage_update_pre<STAGE, EXACT>(st, e + var * B, F[var], L.hv[var], L.dv[var]);
-        }
-    };
-
-    if (it0 >= it1)
+    if (r.it0 >= r.it1)
         return;   // uniform over the workgroup
-    Item cur = item_of(it0);
-    TmemLoads lc = load(cur);
-    to_lds(cur, lc);
-    __syncthreads();
-    for (long it = it0; it < it1; it += W) {
-        const bool more = it + W < it1;
-        Item nxt = cur;
-        TmemLoads ln = lc;
-        if (more) {
-            nxt = item_of(it + W);
-            ln = load(nxt);
-        }
-        if (cur.live)
-            compute(cur, lc);
-        __syncthreads();   // every wave is done reading this item's LDS
-        if (!more)
-            break;
-        to_lds(nxt, ln);
+    // loads are unconditional (a clamped, valid item past the end) so the
+    // wait before each use counts a fixed number of loads
+    auto clamp = [&](long i) { return i < r.it1 ? i : r.it0; };
+    if (wv >= 4) {
+        const TmemItem I0 = tmem_item(tl, r.it0, wid, active_members);
+        const TmemFill f0 = tmem_fill_load<NNBR>(m, tl, st.src, B, I0, wid, lane);
+        TmemItem nI = tmem_item(tl, clamp(r.it0 + r.W), wid, active_members);
+        TmemFill nf = tmem_fill_load<NNBR>(m, tl, st.src, B, nI, wid, lane);
+        tmem_fill_store(lds, I0, f0, wid, lane);
         __syncthreads();
-        cur = nxt;
-        lc = ln;
+        int b = 0;
+        for (long it = r.it0; it < r.it1; it += r.W) {
+            const TmemItem I2 = tmem_item(tl, clamp(it + 2 * r.W), wid, active_members);
+            const TmemFill f2 = tmem_fill_load<NNBR>(m, tl, st.src, B, I2, wid, lane);
+            if (it + r.W < r.it1)
+                tmem_fill_store(lds + (b ^ 1) * kTmemItemDoubles, nI, nf, wid, lane);
+            __syncthreads();   // image b is free, image b ^ 1 is filled
+            nI = I2;
+            nf = f2;
+            b ^= 1;
+        }
+    } else {
+        // the epilogue operands of item i + 1 are loaded while item i is
+        // computed (their HBM latency exceeds one stencil loop)
+        TmemItem I = tmem_item(tl, r.it0, wid, active_members);
+        TmemOwn o = tmem_own_load<STAGE>(st, st.src, B, I, lane);
+        __syncthreads();
+        int b = 0;
+        for (long it = r.it0; it < r.it1; it += r.W) {
+            const TmemItem In = tmem_item(tl, clamp(it + r.W), wid, active_members);
+            const TmemOwn on = tmem_own_load<STAGE>(st, st.src, B, In, lane);
+            if (I.live)
+                tmem_compute<STAGE, EXACT, NNBR, 4>(m, tl, st, B, lds + b * kTmemItemDoubles, I,
+                                                    o, wid, lane);
+            __syncthreads();
+            I = In;
+            o = on;
+            b ^= 1;
+        }
     }
 }
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #define CHECK(x) do { hipError_t e = (x); if (e != hipSuccess) { printf("HIP error %s at %d\n", hipGetErrorString(e), __LINE__); exit(1);} } while (0)
 
@@ -93,6 +94,47 @@ __global__ __launch_bounds__(256) void k_glob(const double *__restrict__ src, do
     out[blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
+// FP64 FMA issue rate at a given occupancy: 16 independent accumulators per
+// lane (the stencil sums of one slot), the weight either an ordinary VGPR or
+// taken from lane k of the 16-lane row by 64-bit DPP row_newbcast.  Each
+// wave records the shader-clock cycles of its loop in out[wave].
+template <int K>
+__device__ __forceinline__ void fmac_bc(double &acc, double w, double x)
+{
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(w), "v"(x), "i"(K));
+}
+template <bool DPP>
+__global__ __launch_bounds__(256) void k_fma64_acc16(double *out, double x0)
+{
+    double s[16];
+    const double w = 1.0 + threadIdx.x * 1e-6;   // written long before any DPP read
+    double x = x0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s[i] = 0.0;
+    const unsigned long long c0 = __builtin_readcyclecounter();
+    for (int it = 0; it < ITERS; ++it) {
+        if constexpr (DPP) {
+            fmac_bc<0>(s[0], w, x); fmac_bc<1>(s[1], w, x); fmac_bc<2>(s[2], w, x); fmac_bc<3>(s[3], w, x);
+            fmac_bc<4>(s[4], w, x); fmac_bc<5>(s[5], w, x); fmac_bc<6>(s[6], w, x); fmac_bc<7>(s[7], w, x);
+            fmac_bc<8>(s[8], w, x); fmac_bc<9>(s[9], w, x); fmac_bc<10>(s[10], w, x); fmac_bc<11>(s[11], w, x);
+            fmac_bc<12>(s[12], w, x); fmac_bc<13>(s[13], w, x); fmac_bc<14>(s[14], w, x); fmac_bc<15>(s[15], w, x);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) s[i] = __builtin_fma(w, x, s[i]);
+        }
+        asm volatile("" : "+v"(x));
+    }
+    const unsigned long long c1 = __builtin_readcyclecounter();
+    double acc = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc += s[i];
+    out[blockIdx.x * 256 + threadIdx.x] = acc;
+    if ((threadIdx.x & 63) == 0)
+        out[gridDim.x * 256 + blockIdx.x * 4 + (threadIdx.x >> 6)] = (double)(c1 - c0);
+}
+
 int main()
 {
     int dev = 0; hipDeviceProp_t p; CHECK(hipGetDeviceProperties(&p, dev));
@@ -141,6 +183,26 @@ int main()
         printf("%-22s %8.3f ms  %8.1f TFLOP/s  %6.2f clk/wave-instr/SIMD\n", "v_mfma_f64_16x16x4", ms,
                instr * 2 * 16 * 16 * 4 / (ms * 1e-3) / 1e12, (ms * 1e-3) * ghz * 1e9 * cus * 4 / instr);
     }
+    // FP64 FMA issue rate with 16 accumulators, 1 and 2 waves per SIMD,
+    // plain and DPP row-broadcast weight: shader-clock cycles per wave
+    // instruction, from each wave's own cycle counter
+    for (int wps = 1; wps <= 2; ++wps)
+        for (int dpp = 0; dpp <= 1; ++dpp) {
+            const int nb = cus * wps;   // 4 waves per block: wps waves per SIMD
+            auto launch = [&] {
+                if (dpp) k_fma64_acc16<true><<<nb, 256>>>(out, 0.5);
+                else k_fma64_acc16<false><<<nb, 256>>>(out, 0.5);
+            };
+            launch(); CHECK(hipDeviceSynchronize());
+            std::vector<double> h((size_t)nb * 4);
+            CHECK(hipMemcpy(h.data(), out + (size_t)nb * 256, sizeof(double) * nb * 4,
+                            hipMemcpyDeviceToHost));
+            double cyc = 0;
+            for (double v : h) cyc += v;
+            cyc /= h.size();
+            printf("v_fmac_f64%-11s %d wave(s)/SIMD  %6.2f clk/wave-instr (wave's own counter)\n",
+                   dpp ? "_dpp bcast" : "", wps, cyc / ((double)ITERS * 16));
+        }
     CHECK(hipFree(out)); CHECK(hipFree(src));
     return 0;
 }

@@ -143,13 +143,13 @@ class HipContext:
     def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0,
                  member_block: int = 0, share: "HipContext | None" = None,
                  kernel: str = "auto", prefetch: int = 0, order: str = "host",
-                 tiled: bool = False, slot_block: int = 0):
+                 tiled: int = 0, slot_block: int = 0):
         self.prob = prob
         self.members = int(members)
         self.exact = bool(exact)
         opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
                              int(member_block), share.handle if share is not None else None,
-                             KERNELS[kernel], int(prefetch), ORDERS[order], int(bool(tiled)),
+                             KERNELS[kernel], int(prefetch), ORDERS[order], int(tiled),
                              int(slot_block))
         h = C.c_void_p()
         if share is not None:

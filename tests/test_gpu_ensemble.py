@@ -104,9 +104,10 @@ def test_set_active_and_move_member(pkg):
         assert np.array_equal(solo.get_state(), got)
 
 
-@pytest.mark.parametrize("B,mb", [(8, 4), (16, 8), (32, 16), (64, 8), (32, 32), (64, 32)])
+@pytest.mark.parametrize("B,mb", [(8, 4), (16, 8), (32, 16), (64, 8), (32, 32), (64, 32),
+                                  (96, 32)])
 def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb):
-    """LDS-staged kernel over a Morton-ordered device copy: bitwise equal to
+    """LDS-staged kernels over a Morton-ordered device copy: bitwise equal to
     the oracle (node order and staging never change a node's arithmetic)."""
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="sfdl")
     print(f"case: tiled exact B={B} mb={mb} N=2562", flush=True)
