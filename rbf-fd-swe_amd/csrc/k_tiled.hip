@@ -13,6 +13,8 @@ DevTiles tiles_of(const swe_hip_ctx *c, int mb)
     tl.lidx = c->tiles.d_lidx;
     tl.lidx_node = c->tiles.d_lidx_node;
     tl.ulist_tile = c->tiles.d_ulist_tile;
+    tl.wts_node = c->tiles.d_wts_node;
+    tl.geo_node = c->tiles.d_geo_node;
     tl.ntiles = c->tiles.ntiles;
     tl.nmb = c->B / mb;
     return tl;

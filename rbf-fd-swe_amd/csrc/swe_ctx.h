@@ -64,6 +64,8 @@ struct SweTiles {
     unsigned short *d_lidx = nullptr;
     unsigned short *d_lidx_node = nullptr;   // [n][32] node-major copy of d_lidx
     int *d_ulist_tile = nullptr;             // [ntiles][kTmemStride] (member-lane kernel)
+    double *d_wts_node = nullptr;            // [n][32][4] (member-lane kernel)
+    double *d_geo_node = nullptr;            // [n][32] (member-lane kernel)
 };
 
 struct swe_hip_ctx {

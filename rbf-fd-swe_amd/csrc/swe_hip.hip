@@ -219,7 +219,8 @@ void free_ctx(swe_hip_ctx *c)
     if (c->graphN) (void)hipGraphExecDestroy(c->graphN);
     void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
-                    c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile};
+                    c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile,
+                    c->tiles.d_wts_node, c->tiles.d_geo_node};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -400,6 +401,35 @@ int build_mesh(const swe_hip_static *s, int device, int order, int sblk, SweMesh
 
 // Greedy tiles along the device node order: a tile grows while it has fewer
 // than tmax nodes and its distinct stencil nodes fit umax LDS rows.
+// Node-major copies of the per-node data a member-lane tile item stages,
+// built once per context: wts_node [n][32][4] (slots past the stencil
+// zero) and geo_node [n][32] (fields 0..16, zeros, the node's 32 16-bit
+// tile positions as doubles 24..31).  Thread = (node, slot).
+__global__ void k_pack_tile_nodes(DevMesh m, int n, const unsigned short *lidx_node,
+                                  double *wts_node, double *geo_node)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 32L * n)
+        return;
+    const int node = (int)(t >> 5), j = (int)(t & 31);
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < m.nnbr) {
+        const double *p = m.wts + 4L * slot_base<0>(m, node) + 4L * j * slot_stride(m);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            w[k] = p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        wts_node[4 * t + k] = w[k];
+    double g = 0.0;
+    if (j < GEO_N)
+        g = m.geo[(long)j * m.ld + node];
+    else if (j >= kTmemPosSlot)
+        g = reinterpret_cast<const double *>(lidx_node + 32L * node)[j - kTmemPosSlot];
+    geo_node[t] = g;
+}
+
 int build_tiles(swe_hip_ctx *c, int tmax, int umax)
 {
     const int n = c->n, W = c->nnbr;
@@ -476,6 +506,16 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
         e = hipMemcpy(t.d_ulist_tile, ulist_tile.data(), sizeof(int) * ulist_tile.size(), hipMemcpyHostToDevice);
     if (!rc && e != hipSuccess)
         rc = fail(SWE_HIP_EDEVICE, std::string("tile upload: ") + hipGetErrorString(e));
+    if (!rc && !ulist_tile.empty()) {
+        rc = dev_alloc(&c->bytes, (void **)&t.d_wts_node, sizeof(double) * 128 * (size_t)n);
+        if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_geo_node, sizeof(double) * 32 * (size_t)n);
+        if (!rc) {
+            const long threads = 32L * n;
+            k_pack_tile_nodes<<<(unsigned)((threads + 255) / 256), 256, 0, c->stream>>>(
+                mesh_of(c), n, t.d_lidx_node, t.d_wts_node, t.d_geo_node);
+            rc = check_launch("k_pack_tile_nodes");
+        }
+    }
     if (rc)
         t.ntiles = 0;
     return rc;

@@ -830,6 +830,8 @@ struct DevTiles {
                                     // zero-padded (member-lane kernel only)
     const unsigned short *lidx;     // [nnbr][ld] position of nbr(j, node) in its tile's list
     const unsigned short *lidx_node;  // [n][32] the same, node-major (64 B per node)
+    const double *wts_node;         // [n][32][4] weights, node-major (member-lane kernel only)
+    const double *geo_node;         // [n][32] geometry 0..16, positions in 24..31 (same)
     int ntiles;
     int nmb;                        // member blocks per tile (B / MB)
 };
@@ -1023,7 +1025,7 @@ constexpr int kTmemItemDoubles = kTmemWtsDoubles + kTmemGeoDoubles + 128 * kTmem
 // Global loads that fill an item's LDS image (wave slice wid's share),
 // held in registers until written.  gx is lane (half, k)'s 8-byte share of
 // its node's geometry row: field k for k < 17, stencil positions
-// 4 (k - 24) .. 4 (k - 24) + 3 for k >= 24 (filler for 17..23).
+// 4 (k - 24) .. 4 (k - 24) + 3 for k >= 24 (zero for 17..23).
 struct TmemFill {
     double2 fx[kTmemFill];
     double2 xa, xb;
@@ -1034,27 +1036,21 @@ struct TmemFill {
 constexpr int kTmemPosSlot = 24;
 static_assert(GEO_N <= kTmemPosSlot, "geometry fields and positions share a 32-double row");
 
+
 template <int NNBR>
 __device__ __forceinline__ TmemFill tmem_fill_load(const DevMesh &m, const DevTiles &tl,
                                                    const double *__restrict__ src, long B,
                                                    const TmemItem &I, int wid, int lane)
 {
-    const long ld = slot_stride(m);
-    const int jw = (lane >> 1) < NNBR ? (lane >> 1) : 0;
-    const int k = lane & 31;
     const int fv = lane >> 4;
     const int fp = lane & 15;
     TmemFill L;
-    L.xa = *reinterpret_cast<const double2 *>(m.wts + 4L * slot_base<NNBR>(m, I.nA) +
-                                              4L * jw * ld + 2 * (lane & 1));
-    L.xb = *reinterpret_cast<const double2 *>(m.wts + 4L * slot_base<NNBR>(m, I.nB) +
-                                              4L * jw * ld + 2 * (lane & 1));
+    // node-major copies: a node's weights are one contiguous 1 KB wave load,
+    // its geometry and positions one 256-byte half-wave load
+    L.xa = *reinterpret_cast<const double2 *>(tl.wts_node + 128L * I.nA + 2 * lane);
+    L.xb = *reinterpret_cast<const double2 *>(tl.wts_node + 128L * I.nB + 2 * lane);
     const int node = (lane >> 5) ? I.nB : I.nA;
-    const double *gsrc =
-        k >= kTmemPosSlot
-            ? reinterpret_cast<const double *>(tl.lidx_node + 32L * node) + (k - kTmemPosSlot)
-            : m.geo + (long)(k < GEO_N ? k : 0) * m.ld + node;
-    L.gx = *gsrc;
+    L.gx = tl.geo_node[32L * node + (lane & 31)];
     const double *__restrict__ s0 = src + I.col0 + 2 * fp + (long)fv * B;
     // one merged scalar load of row ids at a fixed per-tile stride (no wait
     // for the tile's offset); the zero padding past the tile's count is a
