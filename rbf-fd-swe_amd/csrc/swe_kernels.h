@@ -1037,10 +1037,44 @@ constexpr int kTmemPosSlot = 24;
 static_assert(GEO_N <= kTmemPosSlot, "geometry fields and positions share a 32-double row");
 
 
+// An item and its row ids as a loader wave needs them: every scalar load
+// is issued before any is used (one round trip instead of one per group).
+struct TmemLoaderItem {
+    TmemItem I;
+    int ids[kTmemFill];
+};
+
+__device__ __forceinline__ TmemLoaderItem tmem_loader_item(const DevTiles &tl, long it, int wid,
+                                                           int active_members)
+{
+    TmemLoaderItem L;
+    const int tile = (int)(it / tl.nmb);
+    const int *ul = tl.ulist_tile + (long)tile * kTmemStride + wid * kTmemFill;
+#pragma unroll
+    for (int q = 0; q < kTmemFill; ++q)
+        L.ids[q] = c_i32(ul + q);
+    int t0 = c_i32(tl.tile_start + tile), t1 = c_i32(tl.tile_start + tile + 1);
+    int u0 = c_i32(tl.ustart + tile), u1 = c_i32(tl.ustart + tile + 1);
+    asm volatile("" : "+s"(t0), "+s"(t1), "+s"(u0), "+s"(u1));
+    const int mb = (int)(it - (long)tile * tl.nmb);
+    TmemItem &I = L.I;
+    I.tile = tile;
+    I.t0 = t0;
+    I.tcount = t1 - t0;
+    I.ucount = u1 - u0;
+    I.col0 = mb * 32;
+    const int i0 = 2 * wid;
+    I.live = i0 < I.tcount && I.col0 < active_members;
+    I.hasB = i0 + 1 < I.tcount;
+    I.nA = I.t0 + (i0 < I.tcount ? i0 : 0);
+    I.nB = I.hasB ? I.nA + 1 : I.nA;
+    return L;
+}
+
 template <int NNBR>
 __device__ __forceinline__ TmemFill tmem_fill_load(const DevMesh &m, const DevTiles &tl,
                                                    const double *__restrict__ src, long B,
-                                                   const TmemItem &I, int wid, int lane)
+                                                   const TmemItem &I, const int *ids, int lane)
 {
     const int fv = lane >> 4;
     const int fp = lane & 15;
@@ -1052,13 +1086,11 @@ __device__ __forceinline__ TmemFill tmem_fill_load(const DevMesh &m, const DevTi
     const int node = (lane >> 5) ? I.nB : I.nA;
     L.gx = tl.geo_node[32L * node + (lane & 31)];
     const double *__restrict__ s0 = src + I.col0 + 2 * fp + (long)fv * B;
-    // one merged scalar load of row ids at a fixed per-tile stride (no wait
-    // for the tile's offset); the zero padding past the tile's count is a
-    // readable node id, loaded but not stored
-    const int *ul = tl.ulist_tile + (long)I.tile * kTmemStride + wid * kTmemFill;
+    // row ids at a fixed per-tile stride; the zero padding past the tile's
+    // count is a readable node id, loaded but not stored
 #pragma unroll
     for (int q = 0; q < kTmemFill; ++q)
-        L.fx[q] = *reinterpret_cast<const double2 *>(s0 + 4L * c_i32(ul + q) * B);
+        L.fx[q] = *reinterpret_cast<const double2 *>(s0 + 4L * ids[q] * B);
     return L;
 }
 
@@ -1223,16 +1255,20 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
     // wait before each use counts a fixed number of loads
     auto clamp = [&](long i) { return i < r.it1 ? i : r.it0; };
     if (wv >= 4) {
-        const TmemItem I0 = tmem_item(tl, r.it0, wid, active_members);
-        const TmemFill f0 = tmem_fill_load<NNBR>(m, tl, st.src, B, I0, wid, lane);
-        TmemItem nI = tmem_item(tl, clamp(r.it0 + r.W), wid, active_members);
-        TmemFill nf = tmem_fill_load<NNBR>(m, tl, st.src, B, nI, wid, lane);
+        const TmemLoaderItem L0 = tmem_loader_item(tl, r.it0, wid, active_members);
+        const TmemItem I0 = L0.I;
+        const TmemFill f0 = tmem_fill_load<NNBR>(m, tl, st.src, B, I0, L0.ids, lane);
+        const TmemLoaderItem L1 = tmem_loader_item(tl, clamp(r.it0 + r.W), wid, active_members);
+        TmemItem nI = L1.I;
+        TmemFill nf = tmem_fill_load<NNBR>(m, tl, st.src, B, nI, L1.ids, lane);
         tmem_fill_store(lds, I0, f0, wid, lane);
         __syncthreads();
         int b = 0;
         for (long it = r.it0; it < r.it1; it += r.W) {
-            const TmemItem I2 = tmem_item(tl, clamp(it + 2 * r.W), wid, active_members);
-            const TmemFill f2 = tmem_fill_load<NNBR>(m, tl, st.src, B, I2, wid, lane);
+            const TmemLoaderItem L2 = tmem_loader_item(tl, clamp(it + 2 * r.W), wid,
+                                                       active_members);
+            const TmemItem I2 = L2.I;
+            const TmemFill f2 = tmem_fill_load<NNBR>(m, tl, st.src, B, I2, L2.ids, lane);
             if (it + r.W < r.it1)
                 tmem_fill_store(lds + (b ^ 1) * kTmemItemDoubles, nI, nf, wid, lane);
             __syncthreads();   // image b is free, image b ^ 1 is filled
