@@ -57,7 +57,7 @@ def parse_args():
     ap.add_argument("--member-block", type=int, default=0)
     ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw", "pair"])
     ap.add_argument("--prefetch", type=int, default=0)
-    ap.add_argument("--order", default="host", choices=["host", "morton"])
+    ap.add_argument("--order", default="host", choices=["host", "morton", "hilbert"])
     ap.add_argument("--tiled", action="store_true")
     ap.add_argument("--slot-block", type=int, default=0, choices=[0, 16])
     ap.add_argument("--layout", default="cfdl")
@@ -154,7 +154,7 @@ def ensemble_leg(pkg, torch, steps=500):
     single = pkg.runtime.HipContext(prob)
     single.set_state(inits[0])
     ms1 = timed(single, steps)
-    base = pkg.runtime.HipContext(prob, order="morton")
+    base = pkg.runtime.HipContext(prob, order="hilbert")
     candidates = [("member pairs (16-byte gathers), pipeline depth 3",
                    dict(member_block=64, prefetch=0)),
                   ("LDS tiles with member lanes, member block 32, loader and compute waves "
@@ -179,7 +179,7 @@ def ensemble_leg(pkg, torch, steps=500):
     single.close()
     n = prob.n_nodes
     return {"workload": f"N={n}, 64 members (seeded 1e-3 perturbations), {steps} RK4 steps, "
-                        f"{best[0]}, Morton-ordered device copy",
+                        f"{best[0]}, Hilbert-ordered device copy",
             "member_steps_per_s": 64 / (ms * 1e-3), "ms_per_step": ms,
             "gnode_rhs_per_s": n * 4 * 64 / (ms * 1e-3) / 1e9,
             "fp64_tflops": n * 64 * 4 * 1058 / (ms * 1e-3) / 1e12,

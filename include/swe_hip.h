@@ -100,6 +100,7 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
                                    other widths use QUADW)                   */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
+#define SWE_HIP_ORDER_HILBERT 2 /* device node order = 3-D Hilbert curve     */
 typedef struct swe_hip_options {
     int members;
     unsigned flags;

@@ -290,20 +290,63 @@ uint64_t morton3(double x, double y, double z)
     return spread(q(x)) | spread(q(y)) << 1 | spread(q(z)) << 2;
 }
 
+// 3-D Hilbert index of a point of [-1, 1]^3 (21 bits per axis): Skilling's
+// transform of the axes to the transposed index (J. Skilling, "Programming
+// the Hilbert curve", AIP Conf. Proc. 707, 2004), then bit interleaving.
+// Consecutive indices are always neighbouring cells, so a run of nodes in
+// this order is more compact than in Morton order (fewer, fuller tiles).
+uint64_t hilbert3(double x, double y, double z)
+{
+    auto q = [](double a) {
+        double t = (a + 1.0) * 0.5 * 2097151.0;
+        t = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
+        return (uint32_t)t;
+    };
+    uint32_t X[3] = {q(x), q(y), q(z)};
+    const uint32_t M = 1u << 20;
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {   // inverse undo
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i)   // Gray encode
+        X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q)
+            t ^= Q - 1;
+    for (int i = 0; i < 3; ++i)
+        X[i] ^= t;
+    uint64_t key = 0;
+    for (int b = 20; b >= 0; --b)
+        for (int i = 0; i < 3; ++i)
+            key = key << 1 | ((X[i] >> b) & 1u);
+    return key;
+}
+
 int build_mesh(const swe_hip_static *s, int device, int order, int sblk, SweMesh **out)
 {
     *out = nullptr;
     const int n = s->compute_size;
     const HostLayout lay{s->cfdl ? 1 : 0, s->sfdl ? 1 : 0, s->simd};
     const int W = s->Nnbr, pW = s->padded_Nnbr;
-    // Device node order: the host's (RCM) order, or a Morton curve through
-    // the node coordinates (compact tiles for the LDS-staged kernel).  The
-    // arithmetic of every node is the same in either order.
+    // Device node order: the host's (RCM) order, or a Morton or Hilbert
+    // curve through the node coordinates (compact tiles for the LDS-staged
+    // kernels).  The arithmetic of every node is the same in any order.
     std::vector<int> perm, inv;
-    if (order == SWE_HIP_ORDER_MORTON) {
+    if (order == SWE_HIP_ORDER_MORTON || order == SWE_HIP_ORDER_HILBERT) {
         std::vector<std::pair<uint64_t, int>> key(n);
         for (int i = 0; i < n; ++i)
-            key[i] = {morton3(s->x[i], s->y[i], s->z[i]), i};
+            key[i] = {order == SWE_HIP_ORDER_MORTON ? morton3(s->x[i], s->y[i], s->z[i])
+                                                    : hilbert3(s->x[i], s->y[i], s->z[i]),
+                      i};
         std::sort(key.begin(), key.end());
         perm.resize(n);
         inv.resize(n);
@@ -556,8 +599,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "tiled must be 0 or 1");
     if (o->prefetch != 0 && o->prefetch != 4 && o->prefetch != 8)
         return fail(SWE_HIP_EINVAL, "prefetch must be 0, 4 or 8");
-    if (o->node_order != SWE_HIP_ORDER_HOST && o->node_order != SWE_HIP_ORDER_MORTON)
-        return fail(SWE_HIP_EINVAL, "node_order must be SWE_HIP_ORDER_HOST or _MORTON");
+    if (o->node_order < SWE_HIP_ORDER_HOST || o->node_order > SWE_HIP_ORDER_HILBERT)
+        return fail(SWE_HIP_EINVAL, "node_order must be SWE_HIP_ORDER_HOST, _MORTON or _HILBERT");
     if (o->slot_block != 0 && o->slot_block != 16)
         return fail(SWE_HIP_EINVAL, "slot_block must be 0 or 16");
     if (!o->share) {

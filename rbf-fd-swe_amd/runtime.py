@@ -16,7 +16,7 @@ _ip = C.POINTER(C.c_int)
 
 SWE_HIP_EXACT = 1
 KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4}
-ORDERS = {"host": 0, "morton": 1}
+ORDERS = {"host": 0, "morton": 1, "hilbert": 2}
 
 # Every function include/swe_hip.h declares.
 EXPORTS = (

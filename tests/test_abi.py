@@ -75,7 +75,7 @@ def test_bad_options_rejected_before_device(pkg):
     L = pkg.runtime.lib()
     base = dict(members=1, flags=0, device=0, member_block=0, share=None, kernel=0,
                 prefetch=0, node_order=0, tiled=0, slot_block=0)
-    for field, bad in [("kernel", 5), ("kernel", -1), ("prefetch", 2), ("node_order", 2),
+    for field, bad in [("kernel", 5), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
                        ("slot_block", 8), ("members", 0), ("member_block", 3),
                        ("tiled", 2), ("tiled", -1)]:
         o = pkg.runtime.SweHipOptions(**dict(base, **{field: bad}))

@@ -104,17 +104,20 @@ def test_set_active_and_move_member(pkg):
         assert np.array_equal(solo.get_state(), got)
 
 
-@pytest.mark.parametrize("B,mb", [(8, 4), (16, 8), (32, 16), (64, 8), (32, 32), (64, 32),
-                                  (96, 32)])
-def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb):
-    """LDS-staged kernels over a Morton-ordered device copy: bitwise equal to
-    the oracle (node order and staging never change a node's arithmetic)."""
+@pytest.mark.parametrize("B,mb,order", [(8, 4, "morton"), (16, 8, "morton"), (32, 16, "morton"),
+                                        (64, 8, "morton"), (32, 32, "morton"),
+                                        (64, 32, "morton"), (96, 32, "morton"),
+                                        (16, 8, "hilbert"), (64, 32, "hilbert")])
+def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb, order):
+    """LDS-staged kernels over a space-filling-curve-ordered device copy:
+    bitwise equal to the oracle (node order and staging never change a
+    node's arithmetic)."""
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="sfdl")
-    print(f"case: tiled exact B={B} mb={mb} N=2562", flush=True)
+    print(f"case: tiled exact B={B} mb={mb} order={order} N=2562", flush=True)
     H0 = _members(pkg, prob, B)
     orc = pyoracle.Oracle(prob)
     with pkg.runtime.HipContext(prob, members=B, member_block=mb, exact=True,
-                                order="morton", tiled=True) as ctx:
+                                order=order, tiled=True) as ctx:
         for m in range(B):
             ctx.set_state(H0[m], m)
         ctx.rk_steps(4)
@@ -141,12 +144,12 @@ def test_member_lane_tiles_host_order_substeps_exact(pkg, pyoracle):
             assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 3)), m
 
 
-@pytest.mark.parametrize("mb", [16, 32])
-def test_tiled_fast_composition_and_active(pkg, mb):
+@pytest.mark.parametrize("mb,order", [(16, "morton"), (32, "morton"), (32, "hilbert")])
+def test_tiled_fast_composition_and_active(pkg, mb, order):
     prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl")
-    print(f"case: tiled fast B=64 mb={mb} N=10242 vs solo", flush=True)
+    print(f"case: tiled fast B=64 mb={mb} order={order} N=10242 vs solo", flush=True)
     H0 = _members(pkg, prob, 64)
-    with pkg.runtime.HipContext(prob, members=64, member_block=mb, order="morton",
+    with pkg.runtime.HipContext(prob, members=64, member_block=mb, order=order,
                                 tiled=True) as ctx:
         for m in range(64):
             ctx.set_state(H0[m], m)
