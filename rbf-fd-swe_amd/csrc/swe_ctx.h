@@ -88,7 +88,7 @@ struct swe_hip_ctx {
     double *d_stage = nullptr;   // [n*4] staging for member upload/read-back
     int *d_active = nullptr;     // member slots advanced by the ensemble kernel
     SweTiles tiles;              // LDS-staged ensemble kernel (ntiles == 0: unused)
-    hipGraphExec_t graph1 = nullptr, graphN = nullptr;
+    hipGraphExec_t graph1 = nullptr, graphN = nullptr, graphL = nullptr;
     bool timing = false;
     swe_hip_timers acc{};
     long long bytes = 0;         // state buffers

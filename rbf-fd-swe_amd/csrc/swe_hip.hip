@@ -102,7 +102,8 @@ int fail(int code, const std::string &msg)
             return fail(SWE_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kGraphSteps = 8;   // RK steps per captured graph
+constexpr int kGraphSteps = 8;       // RK steps per captured graph (short calls)
+constexpr int kGraphStepsLong = 64;  // and per long-run graph: fewer replay boundaries
 
 }  // namespace
 
@@ -217,6 +218,7 @@ void free_ctx(swe_hip_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
     if (c->graphN) (void)hipGraphExecDestroy(c->graphN);
+    if (c->graphL) (void)hipGraphExecDestroy(c->graphL);
     void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
                     c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile,
@@ -943,7 +945,13 @@ int swe_hip_rk_steps(swe_hip_ctx *c, int nsteps)
         return rc;
     if (!c->graphN && (rc = capture_steps(c, kGraphSteps, &c->graphN)))
         return rc;
+    // all three graphs are captured on the first call, so no later call
+    // (a timed one) pays for a capture
+    if (!c->graphL && (rc = capture_steps(c, kGraphStepsLong, &c->graphL)))
+        return rc;
     int left = nsteps;
+    for (; left >= kGraphStepsLong; left -= kGraphStepsLong)
+        HIPCHK(hipGraphLaunch(c->graphL, c->stream));
     for (; left >= kGraphSteps; left -= kGraphSteps)
         HIPCHK(hipGraphLaunch(c->graphN, c->stream));
     for (; left > 0; --left)
