@@ -1036,6 +1036,34 @@ struct TmemFill {
 constexpr int kTmemPosSlot = 24;
 static_assert(GEO_N <= kTmemPosSlot, "geometry fields and positions share a 32-double row");
 
+// Doubles 17 and 18 of wave slice w's first geometry row (unused by the
+// geometry) carry the header of the workgroup's next item for compute wave
+// w: (nA, nB) and (col0, live | hasB << 1), so a compute wave learns its
+// next nodes from one LDS read instead of a scalar-memory round trip.
+constexpr int kTmemHdrSlot = 17;
+static_assert(GEO_N <= kTmemHdrSlot && kTmemHdrSlot + 2 <= kTmemPosSlot, "header slots");
+
+__device__ __forceinline__ double tmem_hdr_word(const TmemItem &I, int which)
+{
+    const unsigned lo = which ? (unsigned)I.col0 : (unsigned)I.nA;
+    const unsigned hi = which ? (unsigned)((I.live ? 1 : 0) | (I.hasB ? 2 : 0)) : (unsigned)I.nB;
+    return __builtin_bit_cast(double, (unsigned long long)lo | (unsigned long long)hi << 32);
+}
+
+__device__ __forceinline__ TmemItem tmem_hdr_item(const double *img, int wid)
+{
+    const double *h = img + kTmemWtsDoubles + wid * 64 + kTmemHdrSlot;
+    const unsigned long long a = __builtin_bit_cast(unsigned long long, h[0]);
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, h[1]);
+    TmemItem I = {};
+    I.nA = (int)(unsigned)a;
+    I.nB = (int)(unsigned)(a >> 32);
+    I.col0 = (int)(unsigned)b;
+    I.live = (b >> 32) & 1u;
+    I.hasB = (b >> 33) & 1u;
+    return I;
+}
+
 
 // An item and its row ids as a loader wave needs them: every scalar load
 // is issued before any is used (one round trip instead of one per group).
@@ -1095,7 +1123,8 @@ __device__ __forceinline__ TmemFill tmem_fill_load(const DevMesh &m, const DevTi
 }
 
 __device__ __forceinline__ void tmem_fill_store(double *img, const TmemItem &I,
-                                                const TmemFill &L, int wid, int lane)
+                                                const TmemFill &L, const TmemItem &next, int wid,
+                                                int lane)
 {
     double *wl = img;
     double *gl = img + kTmemWtsDoubles;
@@ -1103,7 +1132,9 @@ __device__ __forceinline__ void tmem_fill_store(double *img, const TmemItem &I,
     double2 *dst = reinterpret_cast<double2 *>(wl) + wid * 128 + lane;
     dst[0] = L.xa;    // [wid][0][slot][..]
     dst[64] = L.xb;   // [wid][1][slot][..]
-    gl[wid * 64 + lane] = L.gx;   // [wid][half][geometry 0..16 | positions 24..31]
+    // [wid][half][geometry 0..16 | next-item header 17, 18 (half 0) | positions 24..31]
+    const double hw = tmem_hdr_word(next, lane == kTmemHdrSlot + 1);
+    gl[wid * 64 + lane] = (lane == kTmemHdrSlot || lane == kTmemHdrSlot + 1) ? hw : L.gx;
     // rows [row][var][32 members] as in global memory: lane (var fv, fp)
     // writes members 2 fp, 2 fp + 1 of variable fv, 16 contiguous bytes
     double2 *d0 = reinterpret_cast<double2 *>(rows) + lane;
@@ -1261,7 +1292,7 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
         const TmemLoaderItem L1 = tmem_loader_item(tl, clamp(r.it0 + r.W), wid, active_members);
         TmemItem nI = L1.I;
         TmemFill nf = tmem_fill_load<NNBR>(m, tl, st.src, B, nI, L1.ids, lane);
-        tmem_fill_store(lds, I0, f0, wid, lane);
+        tmem_fill_store(lds, I0, f0, nI, wid, lane);
         __syncthreads();
         int b = 0;
         for (long it = r.it0; it < r.it1; it += r.W) {
@@ -1270,7 +1301,7 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
             const TmemItem I2 = L2.I;
             const TmemFill f2 = tmem_fill_load<NNBR>(m, tl, st.src, B, I2, L2.ids, lane);
             if (it + r.W < r.it1)
-                tmem_fill_store(lds + (b ^ 1) * kTmemItemDoubles, nI, nf, wid, lane);
+                tmem_fill_store(lds + (b ^ 1) * kTmemItemDoubles, nI, nf, I2, wid, lane);
             __syncthreads();   // image b is free, image b ^ 1 is filled
             nI = I2;
             nf = f2;
@@ -1284,7 +1315,8 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
         __syncthreads();
         int b = 0;
         for (long it = r.it0; it < r.it1; it += r.W) {
-            const TmemItem In = tmem_item(tl, clamp(it + r.W), wid, active_members);
+            // the next item's nodes from the header its loader left in this image
+            const TmemItem In = tmem_hdr_item(lds + b * kTmemItemDoubles, wid);
             const TmemOwn on = tmem_own_load<STAGE>(st, st.src, B, In, lane);
             if (I.live)
                 tmem_compute<STAGE, EXACT, NNBR, 4>(m, tl, st, B, lds + b * kTmemItemDoubles, I,
