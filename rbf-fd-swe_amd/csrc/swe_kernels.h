@@ -905,48 +905,12 @@ __global__ __launch_bounds__(512) void k_stage_tiled(DevMesh m, DevTiles tl, Dev
         stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
 }
 
-// Wave-uniform loads through the constant address space (read-only, so a
-// scalar load even after stores the compiler cannot rule out as aliases),
-// pinned to scalar registers: the empty (non-volatile, freely scheduled) asm
-// keeps the compiler from turning "c ? load(b) : load(a)" into one load from
-// a per-lane address.
-__device__ __forceinline__ unsigned s_u32(const unsigned *q)
-{
-    unsigned x = *(const __attribute__((address_space(4))) unsigned *)q;
-    asm("" : "+s"(x));
-    return x;
-}
-__device__ __forceinline__ double s_f64(const double *q)
-{
-    double x = *(const __attribute__((address_space(4))) double *)q;
-    asm("" : "+s"(x));
-    return x;
-}
-// Wave-uniform load with no pin (nothing selects between two of them).
+// Wave-uniform load through the constant address space: read-only, so it
+// stays a scalar load even after stores the compiler cannot rule out as
+// aliases.
 __device__ __forceinline__ int c_i32(const int *q)
 {
     return *(const __attribute__((address_space(4))) int *)q;
-}
-
-// Geometry of one of two wave-uniform nodes (a for the lower half wave, b
-// for the upper), both read with scalar loads.
-__device__ __forceinline__ Geo load_geo_pair(const double *__restrict__ geo, int ld, int a, int b,
-                                             bool upper)
-{
-    double gf[GEO_N];
-#pragma unroll
-    for (int f = 0; f < GEO_N; ++f) {
-        const double xa = s_f64(geo + (long)f * ld + a);
-        const double xb = s_f64(geo + (long)f * ld + b);
-        gf[f] = upper ? xb : xa;
-    }
-    Geo g;
-    g.x = gf[G_X]; g.y = gf[G_Y]; g.z = gf[G_Z]; g.f = gf[G_F]; g.ghm = gf[G_GHM];
-    g.pu0 = gf[G_PU]; g.pu1 = gf[G_PU + 1]; g.pu2 = gf[G_PU + 2];
-    g.pv0 = gf[G_PV]; g.pv1 = gf[G_PV + 1]; g.pv2 = gf[G_PV + 2];
-    g.pw0 = gf[G_PW]; g.pw1 = gf[G_PW + 1]; g.pw2 = gf[G_PW + 2];
-    g.gx = gf[G_GG]; g.gy = gf[G_GG + 1]; g.gz = gf[G_GG + 2];
-    return g;
 }
 
 // ---------------------------------------------------------------------------
