@@ -71,6 +71,9 @@ def parse_args():
                     help="add the serving leg (Poisson mix of forecast requests)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=25.0)
+    ap.add_argument("--serve-kernel", default="tiles", choices=["tiles", "pairs"],
+                    help="batch kernel of the serving leg: member-lane LDS tiles (member "
+                         "block 32) or member pairs (member block 64)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -187,7 +190,7 @@ def ensemble_leg(pkg, torch, steps=500):
             "ratio_vs_single": (64 / ms) / (1 / ms1), "state_finite": finite}
 
 
-def serve_leg(pkg, n_requests, rate, seed=2026):
+def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026):
     """Continuous-batching mix: Poisson arrivals, N in {2562, 10242, 40962,
     163842}, 1-32 members, 50-500 steps, HBM cap 200 GB."""
     import numpy as np
@@ -213,14 +216,16 @@ def serve_leg(pkg, n_requests, rate, seed=2026):
             print(f"bench: serve {done[0]}/{len(reqs)} done", file=sys.stderr, flush=True)
     work = sum(len(r.inits) * r.nsteps for r in reqs)
     node_rhs = sum(len(r.inits) * r.nsteps * 4 * probs[r.mesh].n_nodes for r in reqs)
-    with pkg.serve.Server(hbm_cap_bytes=200 << 30, slots=64, member_block=64,
-                          chunk_steps=16) as srv:
+    tiles = kernel == "tiles"
+    with pkg.serve.Server(hbm_cap_bytes=200 << 30, slots=64, member_block=32 if tiles else 64,
+                          chunk_steps=16, tiled=tiles) as srv:
         for p in probs:
             srv.add_mesh(p)
         stats = srv.run(reqs, on_done=on_done)
     lat = np.array(lat)
     return {"workload": f"{n_requests} requests, Poisson {rate}/s, N in {{2562,10242,40962,163842}}, "
-                        "1-32 members, 50-500 steps, HBM cap 200 GB",
+                        "1-32 members, 50-500 steps, HBM cap 200 GB, batches of 64 slots on "
+                        + ("member-lane LDS tiles" if tiles else "member pairs"),
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "p50_queue_s": float(np.percentile(np.array(que), 50)),
@@ -304,7 +309,7 @@ def main():
         ensemble = ensemble_leg(pkg, torch)
     serving = None
     if args.serve:
-        serving = serve_leg(pkg, args.serve_requests, args.serve_rate)
+        serving = serve_leg(pkg, args.serve_requests, args.serve_rate, args.serve_kernel)
     out = {
         "metric": "eval_RHS GNodes/s + HBM GB/s; ensemble steps/s and p50/p99 latency",
         "value": value,

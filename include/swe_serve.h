@@ -36,6 +36,10 @@ typedef struct swe_srv_config {
     int max_batches_per_mesh;
     unsigned flags;            /* SWE_HIP_EXACT                                 */
     int device;
+    int tiled;                 /* 1: batches of meshes with 31-point stencils use
+                                  the member-lane LDS-tile kernel over a Hilbert-
+                                  ordered copy (member_block 32, slots_per_batch
+                                  a multiple of 32); 0: the member_block kernel */
 } swe_srv_config;
 
 typedef struct swe_srv_stats {

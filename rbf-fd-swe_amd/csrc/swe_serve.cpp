@@ -46,6 +46,8 @@ struct Mesh {
     swe_hip_static desc{};
     long long static_est = 0;    // bytes of the device static data
     long long slot_state_est = 0;
+    long long batch_extra_est = 0;   // per batch context: tile lists and node-major copies
+    bool tiled = false;
     std::vector<int> batches;
     long long static_bytes = 0;  // charged while resident
     bool resident = false;
@@ -124,7 +126,7 @@ int retire(swe_srv *s, Batch &b, int *progress)
 int new_batch(swe_srv *s, int mesh_id, int *batch_id)
 {
     Mesh &m = s->meshes[mesh_id];
-    const long long need = m.slot_state_est * s->cfg.slots_per_batch
+    const long long need = m.slot_state_est * s->cfg.slots_per_batch + m.batch_extra_est
                            + (m.resident ? 0 : m.static_est);
     if (s->hbm_in_use + need > s->cfg.hbm_cap_bytes)
         return 1;   // over budget: not an error, the member waits
@@ -137,7 +139,8 @@ int new_batch(swe_srv *s, int mesh_id, int *batch_id)
             break;
         }
     swe_hip_options o{s->cfg.slots_per_batch, s->cfg.flags, s->cfg.device, s->cfg.member_block,
-                      owner, 0, 0, 0, 0, 0};
+                      owner, 0, 0, m.tiled ? SWE_HIP_ORDER_HILBERT : SWE_HIP_ORDER_HOST,
+                      m.tiled ? 1 : 0, 0};
     swe_hip_ctx *ctx = nullptr;
     int rc = swe_hip_create_ex(owner ? nullptr : &m.desc, &o, &ctx);
     if (rc == SWE_HIP_ENOMEM)
@@ -249,6 +252,7 @@ void swe_srv_default_config(swe_srv_config *c)
     c->max_batches_per_mesh = 64;
     c->flags = 0;
     c->device = 0;
+    c->tiled = 0;
 }
 
 int swe_srv_create(const swe_srv_config *cfg, swe_srv **out)
@@ -256,6 +260,9 @@ int swe_srv_create(const swe_srv_config *cfg, swe_srv **out)
     if (!out || !cfg)
         return SWE_HIP_EINVAL;
     *out = nullptr;
+    if (cfg->tiled != 0 && (cfg->tiled != 1 || cfg->member_block != 32 ||
+                            cfg->slots_per_batch % 32))
+        return SWE_HIP_EINVAL;
     if (cfg->slots_per_batch < 2 || cfg->slots_per_batch % 2 || cfg->chunk_steps < 1
         || cfg->max_batches_per_mesh < 1 || cfg->hbm_cap_bytes <= 0)
         return SWE_HIP_EINVAL;
@@ -285,6 +292,10 @@ int swe_srv_add_mesh(swe_srv *s, const swe_hip_static *d, int *mesh_id)
     const long long n = d->compute_size;
     m.static_est = n * ((long long)d->Nnbr * (4 + 32) + 17 * 8);
     m.slot_state_est = n * 4 * 8 * 5;
+    m.tiled = s->cfg.tiled && d->Nnbr == 31;
+    // tile lists (<= 70 rows per tile, at least one node) and the node-major
+    // weight and geometry copies: at most 2 KB per node
+    m.batch_extra_est = m.tiled ? n * 2048 : 0;
     s->meshes.push_back(m);
     *mesh_id = (int)s->meshes.size() - 1;
     return 0;

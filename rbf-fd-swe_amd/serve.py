@@ -22,7 +22,8 @@ EXPORTS = ("swe_srv_default_config", "swe_srv_create", "swe_srv_destroy", "swe_s
 class SrvConfig(C.Structure):
     _fields_ = [("hbm_cap_bytes", C.c_longlong), ("slots_per_batch", C.c_int),
                 ("member_block", C.c_int), ("chunk_steps", C.c_int),
-                ("max_batches_per_mesh", C.c_int), ("flags", C.c_uint), ("device", C.c_int)]
+                ("max_batches_per_mesh", C.c_int), ("flags", C.c_uint), ("device", C.c_int),
+                ("tiled", C.c_int)]
 
 
 class SrvStats(C.Structure):
@@ -86,7 +87,7 @@ class Request:
 class Server:
     def __init__(self, hbm_cap_bytes: int = 200 << 30, slots: int = 64, member_block: int = 16,
                  chunk_steps: int = 16, max_batches: int = 64, exact: bool = False,
-                 device: int = 0):
+                 device: int = 0, tiled: bool = False):
         cfg = SrvConfig()
         lib().swe_srv_default_config(C.byref(cfg))
         cfg.hbm_cap_bytes = int(hbm_cap_bytes)
@@ -96,6 +97,7 @@ class Server:
         cfg.max_batches_per_mesh = int(max_batches)
         cfg.flags = runtime.SWE_HIP_EXACT if exact else 0
         cfg.device = int(device)
+        cfg.tiled = int(bool(tiled))
         h = C.c_void_p()
         _chk(lib().swe_srv_create(C.byref(cfg), C.byref(h)), "swe_srv_create")
         self._h = h

@@ -137,9 +137,22 @@ def test_serve_library_exports_header(pkg):
     assert L.swe_srv_create(None, C.byref(out)) == -1
     cfg = pkg.serve.SrvConfig()
     L.swe_srv_default_config(C.byref(cfg))
-    assert cfg.hbm_cap_bytes == 200 << 30 and cfg.slots_per_batch == 64
+    assert cfg.hbm_cap_bytes == 200 << 30 and cfg.slots_per_batch == 64 and cfg.tiled == 0
     cfg.slots_per_batch = 3
     assert L.swe_srv_create(C.byref(cfg), C.byref(out)) == -1
+    # tiled batches need member block 32 and a multiple of 32 slots
+    L.swe_srv_default_config(C.byref(cfg))
+    cfg.tiled = 1
+    assert L.swe_srv_create(C.byref(cfg), C.byref(out)) == -1   # member_block 16
+    cfg.member_block = 32
+    cfg.slots_per_batch = 48
+    assert L.swe_srv_create(C.byref(cfg), C.byref(out)) == -1
+    cfg.tiled = 2
+    cfg.slots_per_batch = 64
+    assert L.swe_srv_create(C.byref(cfg), C.byref(out)) == -1
+    cfg.tiled = 1
+    assert L.swe_srv_create(C.byref(cfg), C.byref(out)) == 0
+    L.swe_srv_destroy(out)
 
 
 def test_server_queues_without_device(pkg):

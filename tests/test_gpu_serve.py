@@ -76,3 +76,28 @@ def test_hbm_cap_queues_instead_of_failing(pkg):
     assert stats["requests_done"] == len(reqs)
     assert stats["hbm_peak"] <= one_batch * 1.05
     assert stats["batches"] == 1
+
+
+def test_tiled_batches_equal_solo(pkg):
+    """Batches on the member-lane LDS-tile kernel (Hilbert-ordered copies,
+    member block 32) give every request its solo fast-mode bits."""
+    probs = _probs(pkg)
+    reqs = _mix(pkg, probs, 10, seed=13, max_members=6)
+    print(f"case: serving mix tiled requests={len(reqs)}", flush=True)
+    with pkg.serve.Server(slots=32, member_block=32, chunk_steps=4, max_batches=2,
+                          tiled=True) as srv:
+        for p in probs:
+            srv.add_mesh(p)
+        stats = srv.run(reqs)
+    assert stats["requests_done"] == len(reqs)
+    ctxs = [pkg.runtime.HipContext(p) for p in probs]
+    try:
+        for r in reqs:
+            for m, H0 in enumerate(r.inits):
+                ctx = ctxs[r.mesh]
+                ctx.set_state(H0)
+                ctx.rk_steps(r.nsteps)
+                assert np.array_equal(ctx.get_state(), r.results[m]), (r.tag, m)
+    finally:
+        for c in ctxs:
+            c.close()
