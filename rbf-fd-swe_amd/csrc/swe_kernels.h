@@ -932,7 +932,7 @@ __device__ __forceinline__ int c_i32(const int *q)
 // Per member the sums, the RHS and the stage update are those of
 // k_stage_members (bitwise in exact mode).
 constexpr int kTmemTile = 8;    // nodes per tile: 4 waves x 2 nodes
-constexpr int kTmemRows = 70;   // distinct rows per tile: 70 KB of LDS (+10 KB: 2 workgroups fill 160 KB)
+constexpr int kTmemRows = 70;   // distinct rows per tile: 70 KB (+10 KB): two item images fill 160 KB
 constexpr int kTmemWtsDoubles = 4 * 2 * 32 * 4;   // weights [wave][node][slot][4]
 constexpr int kTmemGeoDoubles = 4 * 2 * 32;       // geometry [wave][node][32]
 constexpr int kTmemFill = (kTmemRows + 3) / 4;    // rows per wave
