@@ -133,6 +133,15 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
 int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, int steps,
                               int *best_mb, double *best_ms_per_step);
 
+/* The same over the LDS-tiled ensemble kernels too: member blocks 4, 8, 16
+ * (one member per lane) and 32 (member-lane tiles, 31-point stencils) with
+ * tiled = 1.  Returns the fastest (member block, tiled) pair; tiles follow
+ * the node order of base's device copy (create base with
+ * SWE_HIP_ORDER_HILBERT for compact tiles).  No counterpart in the
+ * reference (its ensemble runs are separate processes). */
+int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int steps,
+                          int *best_mb, int *best_tiled, double *best_ms_per_step);
+
 /* Per-attempt state upload (main.c:297 + 304-315): H in the host state
  * layout (SVM_LIN_ID, patch_size*4 values) into member slot `member`. */
 int swe_hip_set_state(swe_hip_ctx *ctx, int member, const double *H);

@@ -568,6 +568,74 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
 
 }  // namespace
 
+namespace {
+
+// Milliseconds per fused RK step of one candidate (member block, tiled) on the
+// static data of `base`: one warm-up step (graph capture), then `steps`
+// timed with HIP events on the candidate's stream.
+int time_candidate(swe_hip_ctx *base, int members, unsigned flags, int mb, int tiled, int steps,
+                   double *per_step)
+{
+    swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, tiled, 0};
+    swe_hip_ctx *c = nullptr;
+    int rc = swe_hip_create_ex(nullptr, &o, &c);
+    if (rc)
+        return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0.0f;
+    rc = swe_hip_rk_steps(c, 1);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipEventCreate(&e0);
+    if (!rc && e == hipSuccess) e = hipEventCreate(&e1);
+    if (!rc && e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    if (!rc && e == hipSuccess) rc = swe_hip_rk_steps(c, steps);
+    if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    free_ctx(c);
+    if (!rc && e != hipSuccess)
+        rc = fail(SWE_HIP_EDEVICE, std::string("tune timing: ") + hipGetErrorString(e));
+    if (!rc)
+        *per_step = (double)ms / steps;
+    return rc;
+}
+
+// Every admissible member block (powers of two 2..64 dividing members) on
+// the member-block kernels, and with `with_tiled` also the LDS-tiled ones:
+// member blocks 4, 8, 16 (one member per lane) and 32 (member-lane tiles,
+// 31-point stencils only).  Tiles follow the node order of base's device
+// copy.
+int tune_candidates(swe_hip_ctx *base, int members, unsigned flags, int steps, bool with_tiled,
+                    int *best_mb, int *best_tiled, double *best_ms)
+{
+    *best_mb = 0;
+    *best_tiled = 0;
+    double best = 1e300;
+    for (int tiled = 0; tiled <= (with_tiled ? 1 : 0); ++tiled)
+        for (int mb = 64; mb >= 2; mb >>= 1) {
+            if (members % mb)
+                continue;
+            if (tiled && !((mb >= 4 && mb <= 16) || (mb == 32 && base->nnbr == 31)))
+                continue;
+            double per = 0.0;
+            const int rc = time_candidate(base, members, flags, mb, tiled, steps, &per);
+            if (rc)
+                return rc;
+            if (per < best) {
+                best = per;
+                *best_mb = mb;
+                *best_tiled = tiled;
+            }
+        }
+    if (best_ms)
+        *best_ms = best;
+    return *best_mb ? 0 : fail(SWE_HIP_EINVAL, "members has no admissible member block");
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *swe_hip_strerror(int code)
@@ -723,43 +791,16 @@ int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, in
 {
     if (!base || members < 2 || steps < 1 || !best_mb)
         return fail(SWE_HIP_EINVAL, "tune_member_block: bad arguments");
-    *best_mb = 0;
-    double best = 1e300;
-    for (int mb = 64; mb >= 2; mb >>= 1) {
-        if (members % mb)
-            continue;
-        swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, 0, 0};
-        swe_hip_ctx *c = nullptr;
-        int rc = swe_hip_create_ex(nullptr, &o, &c);
-        if (rc)
-            return rc;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        float ms = 0.0f;
-        rc = swe_hip_rk_steps(c, 1);
-        hipError_t e = hipSuccess;
-        if (!rc) e = hipEventCreate(&e0);
-        if (!rc && e == hipSuccess) e = hipEventCreate(&e1);
-        if (!rc && e == hipSuccess) e = hipEventRecord(e0, c->stream);
-        if (!rc && e == hipSuccess) rc = swe_hip_rk_steps(c, steps);
-        if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
-        if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
-        if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        free_ctx(c);
-        if (!rc && e != hipSuccess)
-            rc = fail(SWE_HIP_EDEVICE, std::string("tune timing: ") + hipGetErrorString(e));
-        if (rc)
-            return rc;
-        const double per = (double)ms / steps;
-        if (per < best) {
-            best = per;
-            *best_mb = mb;
-        }
-    }
-    if (best_ms)
-        *best_ms = best;
-    return *best_mb ? 0 : fail(SWE_HIP_EINVAL, "members has no admissible member block");
+    int tiled = 0;
+    return tune_candidates(base, members, flags, steps, false, best_mb, &tiled, best_ms);
+}
+
+int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int steps,
+                          int *best_mb, int *best_tiled, double *best_ms)
+{
+    if (!base || members < 2 || steps < 1 || !best_mb || !best_tiled)
+        return fail(SWE_HIP_EINVAL, "tune_ensemble: bad arguments");
+    return tune_candidates(base, members, flags, steps, true, best_mb, best_tiled, best_ms);
 }
 
 int swe_hip_info(const swe_hip_ctx *c, int *n_nodes, int *members, int *nnbr, unsigned *flags)

@@ -27,6 +27,7 @@ EXPORTS = (
     "swe_hip_strerror", "swe_hip_last_error", "swe_hip_create_ex", "swe_hip_query",
     "swe_hip_set_active", "swe_hip_move_member", "swe_hip_static_bytes",
     "swe_hip_tune_member_block",
+    "swe_hip_tune_ensemble",
 )
 
 
@@ -73,6 +74,8 @@ def lib() -> C.CDLL:
                                         C.POINTER(vp)]
         L.swe_hip_tune_member_block.argtypes = [vp, C.c_int, C.c_uint, C.c_int,
                                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.swe_hip_tune_ensemble.argtypes = [vp, C.c_int, C.c_uint, C.c_int, C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.swe_hip_query.argtypes = [vp]
         L.swe_hip_set_active.argtypes = [vp, C.c_int]
         L.swe_hip_move_member.argtypes = [vp, C.c_int, C.c_int]
@@ -234,6 +237,18 @@ class HipContext:
                                              C.byref(mb), C.byref(ms)),
              "swe_hip_tune_member_block")
         return mb.value, ms.value
+
+    def tune_ensemble(self, members: int, steps: int = 5, exact: bool = False):
+        """(best member block, tiled, ms per step) over the member-block and
+        LDS-tiled ensemble kernels on this context's mesh and node order."""
+        mb = C.c_int()
+        tiled = C.c_int()
+        ms = C.c_double()
+        _chk(lib().swe_hip_tune_ensemble(self.handle, int(members),
+                                         SWE_HIP_EXACT if exact else 0, int(steps),
+                                         C.byref(mb), C.byref(tiled), C.byref(ms)),
+             "swe_hip_tune_ensemble")
+        return mb.value, bool(tiled.value), ms.value
 
     def set_active(self, active: int) -> None:
         _chk(lib().swe_hip_set_active(self.handle, int(active)), "swe_hip_set_active")

@@ -81,6 +81,24 @@ def test_autotuner_picks_admissible_block(pkg):
             assert np.array_equal(ctx.get_state(5), base.get_state())
 
 
+def test_ensemble_tuner_covers_tiled_kernels(pkg):
+    """The ensemble tuner times the tiled kernels too and its choice runs:
+    the tuned context equals the single-state path bitwise (fast mode)."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl")
+    print("case: ensemble tuner B=64 N=10242 hilbert", flush=True)
+    with pkg.runtime.HipContext(prob, order="hilbert") as base:
+        mb, tiled, ms = base.tune_ensemble(64, steps=3)
+        assert mb in (2, 4, 8, 16, 32, 64) and ms > 0.0
+        assert not tiled or mb in (4, 8, 16, 32)
+        with pkg.runtime.HipContext(prob, members=64, member_block=mb, tiled=tiled,
+                                    share=base) as ctx:
+            ctx.set_state(prob.H_init, 9)
+            ctx.rk_steps(3)
+            base.set_state(prob.H_init)
+            base.rk_steps(3)
+            assert np.array_equal(ctx.get_state(9), base.get_state())
+
+
 def test_set_active_and_move_member(pkg):
     """Slots beyond the active count are skipped; a moved member continues
     exactly as if it had not moved."""
