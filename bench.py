@@ -57,7 +57,9 @@ def parse_args():
     ap.add_argument("--member-block", type=int, default=0)
     ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw", "pair"])
     ap.add_argument("--prefetch", type=int, default=0)
-    ap.add_argument("--order", default="host", choices=["host", "morton", "hilbert"])
+    ap.add_argument("--order", default="hilbert", choices=["host", "morton", "hilbert"],
+                    help="node order of the device copy (results are identical in any order; "
+                         "Hilbert is the fastest measured at N=40962)")
     ap.add_argument("--tiled", action="store_true")
     ap.add_argument("--slot-block", type=int, default=0, choices=[0, 16])
     ap.add_argument("--layout", default="cfdl")

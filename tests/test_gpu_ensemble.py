@@ -186,23 +186,27 @@ def test_tiled_fast_composition_and_active(pkg, mb, order):
         assert np.array_equal(solo.get_state(), got15)
 
 
-@pytest.mark.parametrize("kernel,pf,sblk", [("quad", 0, 0), ("quad", 8, 0), ("lane", 0, 0),
-                                           ("lane", 4, 0), ("quad", 0, 16), ("lane", 4, 16),
-                                           ("quadw", 0, 0), ("quadw", 4, 16), ("pair", 0, 0),
-                                           ("pair", 0, 16)])
-def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk):
+@pytest.mark.parametrize("kernel,pf,sblk,order", [("quad", 0, 0, "morton"), ("quad", 8, 0, "morton"),
+                                                 ("lane", 0, 0, "morton"), ("lane", 4, 0, "morton"),
+                                                 ("quad", 0, 16, "morton"), ("lane", 4, 16, "morton"),
+                                                 ("quadw", 0, 0, "morton"),
+                                                 ("quadw", 4, 16, "morton"),
+                                                 ("pair", 0, 0, "morton"), ("pair", 0, 16, "morton"),
+                                                 ("pair", 0, 0, "hilbert")])
+def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk, order):
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
-    print(f"case: single-state {kernel} prefetch={pf} slot_block={sblk} morton exact", flush=True)
+    print(f"case: single-state {kernel} prefetch={pf} slot_block={sblk} {order} exact", flush=True)
     want = pyoracle.Oracle(prob).run(prob.H_init, 3)
     with pkg.runtime.HipContext(prob, exact=True, kernel=kernel, prefetch=pf,
-                                order="morton", slot_block=sblk) as ctx:
+                                order=order, slot_block=sblk) as ctx:
         ctx.set_state(prob.H_init)
         ctx.rk_steps(3)
         assert np.array_equal(ctx.get_state(), want)
 
 
 @pytest.mark.parametrize("kernel,order,sblk", [("pair", "host", 0), ("pair", "morton", 16),
-                                               ("quadw", "morton", 16), ("lane", "host", 0)])
+                                               ("pair", "hilbert", 0), ("quadw", "morton", 16),
+                                               ("lane", "host", 0)])
 def test_single_state_variants_fast(pkg, pyoracle, kernel, order, sblk):
     """Fast single-state kernels within 1e-12 of the oracle after 50 steps."""
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
