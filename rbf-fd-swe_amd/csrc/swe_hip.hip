@@ -509,13 +509,18 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
         std::sort(cur.begin(), cur.end());
         for (int k = 0; k < (int)cur.size(); ++k)
             pos[cur[k]] = k;
-        for (int i = first; i < node; ++i)
+        for (int i = first; i < node; ++i) {
             for (int j = 0; j < W; ++j) {
                 const unsigned short l = (unsigned short)pos[nbr[(size_t)j * n + i]];
                 lidx[(size_t)j * n + i] = l;
                 if (j < 32)
                     lidx_node[(size_t)32 * i + j] = l;
             }
+            // spare slot 31: the node's own row in its tile (0xffff if its
+            // stencil does not contain it)
+            if (W < 32)
+                lidx_node[(size_t)32 * i + 31] = mark[i] == tile ? (unsigned short)pos[i] : 0xffff;
+        }
         ulist.insert(ulist.end(), cur.begin(), cur.end());
         tstart.push_back(node);
         ustart.push_back((int)ulist.size());

@@ -1110,9 +1110,9 @@ __device__ __forceinline__ void tmem_fill_store(double *img, const TmemItem &I,
     }
 }
 
-// The epilogue's global operands of a lane's (node, member): own state, H, D.
+// The epilogue's global operands of a lane's (node, member): H and D (the
+// own state comes from the tile's LDS rows).
 struct TmemOwn {
-    double ou, ov, ow, oh;
     double hv[4], dv[4];
 };
 
@@ -1123,7 +1123,6 @@ __device__ __forceinline__ TmemOwn tmem_own_load(const DevStage &st, const doubl
     TmemOwn O;
     const int node = (lane >> 5) ? I.nB : I.nA;
     const long e = 4L * node * B + I.col0 + (lane & 31);
-    O.ou = src[e]; O.ov = src[e + B]; O.ow = src[e + 2 * B]; O.oh = src[e + 3 * B];
 #pragma unroll
     for (int var = 0; var < 4; ++var) {
         O.hv[var] = STAGE != ST_RHS ? st.h[e + var * B] : 0.0;
@@ -1194,11 +1193,22 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
     g.pv0 = gp[G_PV]; g.pv1 = gp[G_PV + 1]; g.pv2 = gp[G_PV + 2];
     g.pw0 = gp[G_PW]; g.pw1 = gp[G_PW + 1]; g.pw2 = gp[G_PW + 2];
     g.gx = gp[G_GG]; g.gy = gp[G_GG + 1]; g.gz = gp[G_GG + 2];
+    // own state: the node's row in this tile (position in the spare slot 31
+    // of its position list), else a global load (nodes outside their own
+    // stencil, e.g. padding)
+    const int node = half ? I.nB : I.nA;
+    const long e = 4L * node * B + I.col0 + ml;
+    const unsigned own = p[15] >> 16;
+    double ou, ov, ow, oh;
+    if (own != 0xffffu) {
+        const double *r = lrow + own * 128;
+        ou = r[0]; ov = r[32]; ow = r[64]; oh = r[96];
+    } else {
+        ou = st.src[e]; ov = st.src[e + B]; ow = st.src[e + 2 * B]; oh = st.src[e + 3 * B];
+    }
     double F[4];
-    rhs_point(S, O.ou, O.ov, O.ow, O.oh, g, m.gh0, F);
+    rhs_point(S, ou, ov, ow, oh, g, m.gh0, F);
     if (!half || I.hasB) {
-        const int node = half ? I.nB : I.nA;
-        const long e = 4L * node * B + I.col0 + ml;
 #pragma unroll
         for (int var = 0; var < 4; ++var)
             stage_update_pre<STAGE, EXACT>(st, e + var * B, F[var], O.hv[var], O.dv[var]);
@@ -1236,7 +1246,7 @@ __device__ __forceinline__ TmemRange tmem_range(const DevTiles &tl, int nblk)
 template <int STAGE, bool EXACT, int NNBR>
 __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, DevStage st)
 {
-    static_assert(NNBR > 0 && NNBR <= 32, "node-major positions hold 32 slots");
+    static_assert(NNBR > 0 && NNBR <= 31, "node-major positions: 31 slots + the own row");
     extern __shared__ double lds[];   // two item images
     const TmemRange r = tmem_range(tl, st.nblk);
     const long B = st.members;
