@@ -938,23 +938,6 @@ constexpr int kTmemGeoDoubles = 4 * 2 * 32;       // geometry [wave][node][32]
 constexpr int kTmemFill = (kTmemRows + 3) / 4;    // rows per wave
 constexpr int kTmemStride = 4 * kTmemFill;        // per-tile stride of the fixed-stride row ids
 
-template <int STAGE, bool EXACT>
-__device__ __forceinline__ void stage_update_pre(const DevStage &st, long e, double Fv,
-                                                 double hv, double dv)
-{
-    if constexpr (STAGE == ST_RHS) {
-        st.out[e] = Fv;
-    } else if constexpr (STAGE == ST_0) {
-        st.d[e] = Fv;
-        st.out[e] = axpy<EXACT>(hv, st.c, Fv);
-    } else if constexpr (STAGE == ST_1 || STAGE == ST_2) {
-        st.d[e] = axpy<EXACT>(dv, st.dcoef, Fv);
-        st.out[e] = axpy<EXACT>(hv, st.c, Fv);
-    } else {
-        st.out[e] = axpy<EXACT>(hv, st.c, axpy<EXACT>(dv, st.dcoef, Fv));
-    }
-}
-
 // One (tile, member block) item as wave slice `wid` (0..3) sees it: the
 // wave owns tile nodes 2 wid, 2 wid + 1 (nA, nB) and copies tile rows
 // [wid * kTmemFill, (wid + 1) * kTmemFill).  All fields wave-uniform.
@@ -1110,25 +1093,58 @@ __device__ __forceinline__ void tmem_fill_store(double *img, const TmemItem &I,
     }
 }
 
-// The epilogue's global operands of a lane's (node, member): H and D (the
-// own state comes from the tile's LDS rows).
+// The epilogue's global operands H and D (the own state comes from the
+// tile's LDS rows) in the epilogue's pair layout: lanes 2p, 2p + 1 of a
+// half-wave hold members 2p, 2p + 1 of variables (0, 1) and (2, 3)
+// respectively, so every load and store is 16 bytes.
 struct TmemOwn {
-    double hv[4], dv[4];
+    double2 hv[2], dv[2];
 };
+
+__device__ __forceinline__ long tmem_pair_elem(long B, int node, int col0, int lane)
+{
+    const int ml = lane & 31;
+    return 4L * node * B + col0 + (ml & ~1) + 2L * (ml & 1) * B;
+}
 
 template <int STAGE>
 __device__ __forceinline__ TmemOwn tmem_own_load(const DevStage &st, const double *__restrict__ src,
                                                  long B, const TmemItem &I, int lane)
 {
     TmemOwn O;
-    const int node = (lane >> 5) ? I.nB : I.nA;
-    const long e = 4L * node * B + I.col0 + (lane & 31);
+    const long e = tmem_pair_elem(B, (lane >> 5) ? I.nB : I.nA, I.col0, lane);
 #pragma unroll
-    for (int var = 0; var < 4; ++var) {
-        O.hv[var] = STAGE != ST_RHS ? st.h[e + var * B] : 0.0;
-        O.dv[var] = (STAGE != ST_RHS && STAGE != ST_0) ? st.d[e + var * B] : 0.0;
+    for (int q = 0; q < 2; ++q) {
+        O.hv[q] = STAGE != ST_RHS ? *reinterpret_cast<const double2 *>(st.h + e + q * B)
+                                  : make_double2(0.0, 0.0);
+        O.dv[q] = (STAGE != ST_RHS && STAGE != ST_0)
+                      ? *reinterpret_cast<const double2 *>(st.d + e + q * B)
+                      : make_double2(0.0, 0.0);
     }
     return O;
+}
+
+// The fused stage update (D, and K or H) of two consecutive elements from
+// prefetched H and D operands (16-byte stores).
+template <int STAGE, bool EXACT>
+__device__ __forceinline__ void stage_update_pre_pair(const DevStage &st, long e, double2 F,
+                                                      double2 hv, double2 dv)
+{
+    double2 *out = reinterpret_cast<double2 *>(st.out + e);
+    if constexpr (STAGE == ST_RHS) {
+        *out = F;
+    } else if constexpr (STAGE == ST_0) {
+        *reinterpret_cast<double2 *>(st.d + e) = F;
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, F.x), axpy<EXACT>(hv.y, st.c, F.y));
+    } else if constexpr (STAGE == ST_1 || STAGE == ST_2) {
+        *reinterpret_cast<double2 *>(st.d + e) =
+            make_double2(axpy<EXACT>(dv.x, st.dcoef, F.x), axpy<EXACT>(dv.y, st.dcoef, F.y));
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, F.x), axpy<EXACT>(hv.y, st.c, F.y));
+    } else {
+        const double d0 = axpy<EXACT>(dv.x, st.dcoef, F.x);
+        const double d1 = axpy<EXACT>(dv.y, st.dcoef, F.y);
+        *out = make_double2(axpy<EXACT>(hv.x, st.c, d0), axpy<EXACT>(hv.y, st.c, d1));
+    }
 }
 
 // Stencil sums of wave slice wid's two nodes from an item's LDS image, then
@@ -1208,10 +1224,18 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
     }
     double F[4];
     rhs_point(S, ou, ov, ow, oh, g, m.gh0, F);
+    // to the pair layout: lane 2p keeps variables 0, 1 and takes member
+    // 2p + 1's; lane 2p + 1 keeps 2, 3 and takes member 2p's (one DPP swap
+    // of two values, every lane active)
+    const int k = ml & 1;
+    const double r0 = pair_swap(k ? F[0] : F[2]);
+    const double r1 = pair_swap(k ? F[1] : F[3]);
+    const double2 Fa = k ? make_double2(r0, F[2]) : make_double2(F[0], r0);
+    const double2 Fb = k ? make_double2(r1, F[3]) : make_double2(F[1], r1);
     if (!half || I.hasB) {
-#pragma unroll
-        for (int var = 0; var < 4; ++var)
-            stage_update_pre<STAGE, EXACT>(st, e + var * B, F[var], O.hv[var], O.dv[var]);
+        const long e2 = tmem_pair_elem(B, node, I.col0, lane);
+        stage_update_pre_pair<STAGE, EXACT>(st, e2, Fa, O.hv[0], O.dv[0]);
+        stage_update_pre_pair<STAGE, EXACT>(st, e2 + B, Fb, O.hv[1], O.dv[1]);
     }
 }
 
