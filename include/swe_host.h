@@ -119,8 +119,9 @@ typedef struct swe_timers {
 double swe_get_time(void);
 int swe_timers_init(swe_timers *t, int nattempts);
 void swe_timers_free(swe_timers *t);
-/* process_profiling_results: per-phase average/min/max/"stddev" in seconds
- * per timestep; Eval_Rhs excludes the first two attempts (profiling.c:120).
+/* process_profiling_results (profiling.c:54-172): per-phase
+ * average/min/max/"stddev" in seconds per timestep; Eval_Rhs excludes the
+ * first two attempts (profiling.c:119-125; all zeros with nattempts <= 2).
  * out[phase*4 + {0,1,2,3}] = avg, min, max, stddev for phases
  * main, eval_rhs, eval_K, update_D, update_H.  Prints the report if
  * print != 0. */

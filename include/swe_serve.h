@@ -48,6 +48,7 @@ typedef struct swe_srv_stats {
     long long hbm_in_use, hbm_peak;
     int batches, meshes_resident;
     long long chunks_launched;
+    long long evictions;       /* idle batches destroyed to make room          */
 } swe_srv_stats;
 
 void swe_srv_default_config(swe_srv_config *cfg);

@@ -188,3 +188,19 @@ def ref_rcm(idx: np.ndarray, layout: str = "cfdl") -> np.ndarray:
     out = np.empty(idx.shape[0], dtype=np.int32)
     L.ref_rcm_of(idx.shape[0], idx.shape[1], idx.ctypes.data_as(_ip), out.ctypes.data_as(_ip))
     return out
+
+
+def ref_profiling(phase_times: dict, nsteps: int, layout: str = "cfdl") -> str:
+    """The reference's process_profiling_results report (its stdout text) for
+    per-attempt phase times (seconds; keys t_main, t_eval_rhs, t_eval_K,
+    t_update_D, t_update_H)."""
+    so = os.path.join(HERE, "_ref", f"libswe_ref_{layout}.so")
+    L = C.CDLL(so)
+    L.ref_profiling.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_char_p, C.c_int]
+    keys = ("t_main", "t_eval_rhs", "t_eval_K", "t_update_D", "t_update_H")
+    arrs = [np.ascontiguousarray(phase_times[k], dtype=np.float64) for k in keys]
+    buf = C.create_string_buffer(1 << 16)
+    n = L.ref_profiling(len(arrs[0]), int(nsteps), *[_ptr(a) for a in arrs], buf, 1 << 16)
+    if n < 0:
+        raise RuntimeError("ref_profiling could not capture the report")
+    return buf.value.decode()

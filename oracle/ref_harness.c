@@ -20,8 +20,10 @@
 #include <rk4_rbffd_swe.h>
 #include <profiling.h>
 
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 sim_params_struct sim_params;
 PSMD_struct *LPSMD;
@@ -150,4 +152,41 @@ void ref_rcm_of(int N, int W, const int *idx, int *out)
     int *m = rcm_mapping(G);
     memcpy(out, m, sizeof(int) * N);
     free(m);
+}
+
+/* process_profiling_results (profiling.c:54-172) on caller-given per-attempt
+ * phase times (seconds): its report, printed to stdout, is captured into
+ * buf (at most cap - 1 bytes, NUL-terminated).  Returns the report length,
+ * or -1 when stdout cannot be redirected. */
+int ref_profiling(int nattempts, int nsteps, const double *t_main, const double *t_eval_rhs,
+                  const double *t_eval_K, const double *t_update_D, const double *t_update_H,
+                  char *buf, int cap)
+{
+    sim_params.nattempts = nattempts;
+    sim_params.nsteps = nsteps;
+    sim_params.OCL = 0;
+    local_timer = init_timer(nattempts);
+    memcpy(local_timer.t_main, t_main, sizeof(double) * nattempts);
+    memcpy(local_timer.t_eval_rhs, t_eval_rhs, sizeof(double) * nattempts);
+    memcpy(local_timer.t_eval_K, t_eval_K, sizeof(double) * nattempts);
+    memcpy(local_timer.t_update_D, t_update_D, sizeof(double) * nattempts);
+    memcpy(local_timer.t_update_H, t_update_H, sizeof(double) * nattempts);
+    FILE *tmp = tmpfile();
+    if (!tmp)
+        return -1;
+    fflush(stdout);
+    const int saved = dup(1);
+    if (saved < 0 || dup2(fileno(tmp), 1) < 0) {
+        fclose(tmp);
+        return -1;
+    }
+    process_profiling_results();
+    fflush(stdout);
+    dup2(saved, 1);
+    close(saved);
+    rewind(tmp);
+    const size_t got = fread(buf, 1, (size_t)(cap - 1), tmp);
+    buf[got] = '\0';
+    fclose(tmp);
+    return (int)got;
 }

@@ -19,6 +19,7 @@ constexpr int kBlock = 256;            // threads per workgroup, stage kernels
 constexpr int kSingleBlock = 64;
 constexpr int kTiledBlock = 512;       // threads per workgroup, tiled kernel
 constexpr int kTiledLds = 80 * 1024;   // LDS per tiled workgroup: two per CU
+constexpr int kGraphCount = 4;         // captured step graphs per context
 
 // Host index macros of the reference (swe_config.h:21-54) at run time.
 struct HostLayout {
@@ -88,7 +89,7 @@ struct swe_hip_ctx {
     double *d_stage = nullptr;   // [n*4] staging for member upload/read-back
     int *d_active = nullptr;     // member slots advanced by the ensemble kernel
     SweTiles tiles;              // LDS-staged ensemble kernel (ntiles == 0: unused)
-    hipGraphExec_t graph1 = nullptr, graphN = nullptr, graphL = nullptr;
+    hipGraphExec_t graphs[swe::kGraphCount] = {};   // 64, 16, 4 and 1 RK steps
     bool timing = false;
     swe_hip_timers acc{};
     long long bytes = 0;         // state buffers

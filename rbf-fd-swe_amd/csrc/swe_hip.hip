@@ -102,8 +102,10 @@ int fail(int code, const std::string &msg)
             return fail(SWE_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kGraphSteps = 8;       // RK steps per captured graph (short calls)
-constexpr int kGraphStepsLong = 64;  // and per long-run graph: fewer replay boundaries
+// RK steps per captured graph: a call of n steps replays the 64-step graph
+// n / 64 times and then at most three of each shorter one (a replay
+// boundary costs a few microseconds)
+constexpr int kGraphSteps[kGraphCount] = {64, 16, 4, 1};
 
 }  // namespace
 
@@ -216,9 +218,8 @@ void free_ctx(swe_hip_ctx *c)
         return;
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->graph1) (void)hipGraphExecDestroy(c->graph1);
-    if (c->graphN) (void)hipGraphExecDestroy(c->graphN);
-    if (c->graphL) (void)hipGraphExecDestroy(c->graphL);
+    for (hipGraphExec_t g : c->graphs)
+        if (g) (void)hipGraphExecDestroy(g);
     void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
                     c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile,
@@ -739,8 +740,10 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
             rc = fail(SWE_HIP_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
     };
     check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // F (the RHS array of the drop-in path) is allocated on first use: the
+    // fused stages never store the RHS
     const size_t sbytes = sizeof(double) * (size_t)state_count(c);
-    double **bufs[] = {&c->d_H, &c->d_Ka, &c->d_Kb, &c->d_D, &c->d_F};
+    double **bufs[] = {&c->d_H, &c->d_Ka, &c->d_Kb, &c->d_D};
     for (double **b : bufs)
         if (!rc) rc = dev_alloc(&c->bytes, (void **)b, sbytes);
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&c->d_stage, sizeof(double) * 4 * (size_t)c->n);
@@ -925,11 +928,29 @@ int swe_hip_get_state(swe_hip_ctx *c, int member, double *H)
     return from_device_member(c, member, c->d_H, H);
 }
 
+// The RHS buffer F of the drop-in path, allocated (and zeroed) on its first
+// use.
+static int ensure_F(swe_hip_ctx *c)
+{
+    if (c->d_F)
+        return 0;
+    HIPCHK(hipSetDevice(c->dev));
+    const size_t sbytes = sizeof(double) * (size_t)state_count(c);
+    int rc = dev_alloc(&c->bytes, (void **)&c->d_F, sbytes);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemsetAsync(c->d_F, 0, sbytes, c->stream));
+    return 0;
+}
+
 int swe_hip_eval_rhs(swe_hip_ctx *c, int member, const double *K, double *F)
 {
     if (!c || !K || !F || member < 0 || member >= (c ? c->B : 0))
         return fail(SWE_HIP_EINVAL, "eval_rhs: bad ctx, pointer or member");
-    int rc = to_device_member(c, member, K, c->d_Kb);
+    int rc = ensure_F(c);
+    if (rc)
+        return rc;
+    rc = to_device_member(c, member, K, c->d_Kb);
     if (rc)
         return rc;
     launch_rhs(c, c->d_Kb, c->d_F);
@@ -944,6 +965,8 @@ int swe_hip_rk_substep(swe_hip_ctx *c, int s)
     if (!c || s < 0 || s > 3)
         return fail(SWE_HIP_EINVAL, "rk_substep: bad ctx or substep_id");
     HIPCHK(hipSetDevice(c->dev));
+    if (int rc0 = ensure_F(c))
+        return rc0;
     const long cnt = state_count(c);
     const int eb = elementwise_blocks(cnt);
     const double dt = c->dt;
@@ -986,22 +1009,16 @@ int swe_hip_rk_steps(swe_hip_ctx *c, int nsteps)
     if (!c || nsteps < 0)
         return fail(SWE_HIP_EINVAL, "rk_steps: bad ctx or nsteps");
     HIPCHK(hipSetDevice(c->dev));
-    int rc = 0;
-    if (!c->graph1 && (rc = capture_steps(c, 1, &c->graph1)))
-        return rc;
-    if (!c->graphN && (rc = capture_steps(c, kGraphSteps, &c->graphN)))
-        return rc;
-    // all three graphs are captured on the first call, so no later call
-    // (a timed one) pays for a capture
-    if (!c->graphL && (rc = capture_steps(c, kGraphStepsLong, &c->graphL)))
-        return rc;
+    // every graph is captured on the first call, so no later (timed) call
+    // pays for a capture
+    for (int g = 0; g < kGraphCount; ++g)
+        if (!c->graphs[g])
+            if (int rc = capture_steps(c, kGraphSteps[g], &c->graphs[g]))
+                return rc;
     int left = nsteps;
-    for (; left >= kGraphStepsLong; left -= kGraphStepsLong)
-        HIPCHK(hipGraphLaunch(c->graphL, c->stream));
-    for (; left >= kGraphSteps; left -= kGraphSteps)
-        HIPCHK(hipGraphLaunch(c->graphN, c->stream));
-    for (; left > 0; --left)
-        HIPCHK(hipGraphLaunch(c->graph1, c->stream));
+    for (int g = 0; g < kGraphCount; ++g)
+        for (; left >= kGraphSteps[g]; left -= kGraphSteps[g])
+            HIPCHK(hipGraphLaunch(c->graphs[g], c->stream));
     return 0;
 }
 

@@ -80,7 +80,7 @@ struct swe_srv {
     std::deque<int> pending;      // requests with members not yet admitted
     std::deque<Done> done;
     long long hbm_in_use = 0, hbm_peak = 0;
-    long long member_steps = 0, chunks = 0, requests_done = 0;
+    long long member_steps = 0, chunks = 0, requests_done = 0, evictions = 0;
 };
 
 namespace {
@@ -123,15 +123,52 @@ int retire(swe_srv *s, Batch &b, int *progress)
     return 0;
 }
 
+// Destroys an idle batch (no live member, no chunk in flight) and returns
+// its bytes to the budget; the mesh's static data go too when this was the
+// mesh's last batch (the device copy is freed with its last context).
+void evict(swe_srv *s, int batch_id)
+{
+    Batch &b = s->batches[batch_id];
+    Mesh &m = s->meshes[b.mesh];
+    swe_hip_destroy(b.ctx);
+    b.ctx = nullptr;
+    s->hbm_in_use -= b.bytes;
+    b.bytes = 0;
+    m.batches.erase(std::find(m.batches.begin(), m.batches.end(), batch_id));
+    if (m.batches.empty() && m.resident) {
+        s->hbm_in_use -= m.static_bytes;
+        m.static_bytes = 0;
+        m.resident = false;
+    }
+    s->evictions++;
+}
+
 int new_batch(swe_srv *s, int mesh_id, int *batch_id)
 {
     Mesh &m = s->meshes[mesh_id];
-    const long long need = m.slot_state_est * s->cfg.slots_per_batch + m.batch_extra_est
-                           + (m.resident ? 0 : m.static_est);
-    if (s->hbm_in_use + need > s->cfg.hbm_cap_bytes)
-        return 1;   // over budget: not an error, the member waits
+    auto need = [&] {
+        return m.slot_state_est * s->cfg.slots_per_batch + m.batch_extra_est
+               + (m.resident ? 0 : m.static_est);
+    };
     if ((int)m.batches.size() >= s->cfg.max_batches_per_mesh)
         return 1;
+    if (s->hbm_in_use + need() > s->cfg.hbm_cap_bytes) {
+        // make room from idle batches of other meshes, oldest first
+        for (int bi = 0; bi < (int)s->batches.size() && s->hbm_in_use + need() >
+                                                            s->cfg.hbm_cap_bytes; ++bi) {
+            const Batch &b = s->batches[bi];
+            if (b.ctx && !b.busy && b.active == 0 && b.mesh != mesh_id)
+                evict(s, bi);
+        }
+    }
+    if (s->hbm_in_use + need() > s->cfg.hbm_cap_bytes) {
+        bool will_free = false;   // a live member or a chunk in flight
+        for (const Batch &b : s->batches)
+            will_free = will_free || (b.ctx && (b.busy || b.active > 0));
+        if (!will_free)
+            return SWE_HIP_ENOMEM;   // the batch can never fit the budget
+        return 1;   // over budget: not an error, the member waits
+    }
     swe_hip_ctx *owner = nullptr;
     for (int bi : m.batches)
         if (s->batches[bi].ctx) {
@@ -291,7 +328,7 @@ int swe_srv_add_mesh(swe_srv *s, const swe_hip_static *d, int *mesh_id)
     m.desc = *d;
     const long long n = d->compute_size;
     m.static_est = n * ((long long)d->Nnbr * (4 + 32) + 17 * 8);
-    m.slot_state_est = n * 4 * 8 * 5;
+    m.slot_state_est = n * 4 * 8 * 4;   // H, K (two), D per member slot
     m.tiled = s->cfg.tiled && d->Nnbr == 31;
     // tile lists (<= 70 rows per tile, at least one node) and the node-major
     // weight and geometry copies: at most 2 KB per node
@@ -414,6 +451,7 @@ int swe_srv_get_stats(const swe_srv *s, swe_srv_stats *st)
     st->batches = nb;
     st->meshes_resident = nm;
     st->chunks_launched = s->chunks;
+    st->evictions = s->evictions;
     return 0;
 }
 

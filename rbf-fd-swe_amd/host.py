@@ -46,6 +46,29 @@ class ProblemStruct(C.Structure):
                 ("H_init", _dp), ("H_100", _dp), ("mapping", _ip)]
 
 
+class Timers(C.Structure):
+    _fields_ = [("nattempts", C.c_int), ("attempt", C.c_int),
+                ("t_init", C.c_double), ("t_dev", C.c_double),
+                ("t_main", _dp), ("t_eval_rhs", _dp), ("t_eval_K", _dp),
+                ("t_update_D", _dp), ("t_update_H", _dp)]
+
+
+PHASES = ("t_main", "t_eval_rhs", "t_eval_K", "t_update_D", "t_update_H")
+
+
+def process_profiling(phase_times: dict, nsteps: int) -> np.ndarray:
+    """swe_process_profiling over per-attempt phase times (seconds; keys
+    PHASES, equal lengths): [5 phases][avg, min, max, stddev] s/step."""
+    arrs = [np.ascontiguousarray(phase_times[k], dtype=np.float64) for k in PHASES]
+    t = Timers()
+    t.nattempts = len(arrs[0])
+    for k, a in zip(PHASES, arrs):
+        setattr(t, k, a.ctypes.data_as(_dp))
+    out = np.zeros(20)
+    lib().swe_process_profiling(C.byref(t), int(nsteps), 0, out.ctypes.data_as(_dp))
+    return out.reshape(5, 4)
+
+
 LAYOUTS = {"default": (0, 0), "cfdl": (1, 0), "sfdl": (0, 1), "cfdl_sfdl": (1, 1)}
 
 _lib = None
@@ -72,6 +95,8 @@ def lib() -> C.CDLL:
         L.swe_vect_lin_id.argtypes = [C.POINTER(Layout), C.c_int, C.c_int]
         L.swe_vect_lin_id.restype = C.c_long
         L.swe_stencil_bandwidth.argtypes = [C.POINTER(Gsmd), _ip, _ip]
+        L.swe_process_profiling.argtypes = [C.POINTER(Timers), C.c_int, C.c_int, _dp]
+        L.swe_process_profiling.restype = None
         _lib = L
     return _lib
 

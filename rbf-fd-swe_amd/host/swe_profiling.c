@@ -2,8 +2,11 @@
  * Host timing contract of the RBF-FD SWE driver (swe_code/main/profiling.c):
  * wall-clock accumulators per attempt, reported in seconds per timestep
  * with average / min / max / "stddev" per phase, where the stddev is the
- * reference's sqrt(sum((avg - x)^2)) / nattempts (profiling.c:129-165) and
- * Eval_Rhs averages only attempts >= 2 (profiling.c:119-125).
+ * reference's sqrt(sum((avg - x)^2)) / nattempts over every attempt
+ * (profiling.c:129-165) and Eval_Rhs averages, minimises and maximises only
+ * attempts >= 2 (profiling.c:119-125): with nattempts <= 2 its Average, Min
+ * and Max are 0 and its STDDEV is taken against that 0.  nsteps <= 0 is
+ * reported per single step (the reference would divide by zero).
  */
 #include "swe_host.h"
 
@@ -54,9 +57,9 @@ static void phase_stats(const double *x, int n, int first, int nsteps, double ou
     }
     for (int i = 0; i < n; i++) {
         const double v = x[i] / nsteps;
-        ss += (avg - v) * (avg - v);
+        ss += pow(avg - v, 2);
     }
-    out[0] = used > 0 ? avg : 0.0;
+    out[0] = avg;
     out[1] = mn;
     out[2] = mx;
     out[3] = sqrt(ss) / n;
@@ -70,7 +73,7 @@ void swe_process_profiling(const swe_timers *t, int nsteps, int print, double ou
     double local[20];
     double *o = out ? out : local;
     for (int k = 0; k < 5; k++) {
-        const int first = (k == 1 && t->nattempts > 2) ? 2 : 0;
+        const int first = k == 1 ? 2 : 0;   /* warm-up attempts out of Eval_Rhs */
         phase_stats(ph[k], t->nattempts, first, nsteps > 0 ? nsteps : 1, &o[4 * k]);
     }
     if (!print)

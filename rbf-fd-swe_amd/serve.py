@@ -30,7 +30,7 @@ class SrvStats(C.Structure):
     _fields_ = [("member_steps_done", C.c_longlong), ("requests_done", C.c_longlong),
                 ("requests_pending", C.c_longlong), ("hbm_in_use", C.c_longlong),
                 ("hbm_peak", C.c_longlong), ("batches", C.c_int), ("meshes_resident", C.c_int),
-                ("chunks_launched", C.c_longlong)]
+                ("chunks_launched", C.c_longlong), ("evictions", C.c_longlong)]
 
 
 _lib = None

@@ -67,7 +67,7 @@ def test_hbm_cap_queues_instead_of_failing(pkg):
     reqs = _mix(pkg, probs, 6, seed=9, max_members=4)
     prob = probs[0]
     n = prob.compute_size
-    one_batch = n * (31 * 36 + 17 * 8) + 4 * 8 * 5 * n * 4 + 4 * 8 * n + 64
+    one_batch = n * (31 * 36 + 17 * 8) + 4 * 8 * 4 * n * 4 + 4 * 8 * n + 64
     with pkg.serve.Server(hbm_cap_bytes=int(one_batch * 1.05), slots=4, member_block=2,
                           chunk_steps=3) as srv:
         srv.add_mesh(prob)

@@ -75,3 +75,37 @@ def test_fast_oracle_close_to_strict(pkg, pyoracle):
     b = pyoracle.Oracle(prob, "fast").run(prob.H_init, 100)
     from conftest import rel_by_var
     assert (rel_by_var(b, a, prob) < 1e-12).all()
+
+
+def _report_rows(text):
+    """[phase][avg, min, max, stddev] from a profiling report's five
+    '(seconds/timestep)' lines, in report order."""
+    import re
+    rows = []
+    for line in text.splitlines():
+        if "(seconds/timestep)" in line:
+            rows.append([float(v) for v in re.findall(r"[-+]?\d\.\d+e[-+]\d+", line)])
+    return rows
+
+
+@pytest.mark.parametrize("nattempts", [1, 2, 3, 4, 7])
+def test_profiling_report_matches_reference(pkg, pyoracle, nattempts):
+    """swe_process_profiling against the reference's process_profiling_results
+    (profiling.c:54-172) on the same per-attempt timers: the warm-up rule
+    (Eval_Rhs skips attempts 0 and 1, all zeros with <= 2 attempts) and the
+    STDDEV = sqrt(sum (avg - x)^2) / nattempts formula.  The timer values are
+    chosen so that including the warm-up attempts, or dividing by sqrt(n),
+    changes the printed digits."""
+    if not pyoracle.ref_available("cfdl"):
+        pytest.skip("reference build missing")
+    rng = np.random.default_rng(nattempts)
+    times = {k: rng.uniform(0.5, 2.0, nattempts) for k in pkg.host.PHASES}
+    times["t_eval_rhs"][:2] *= 10.0   # warm-up attempts are slow
+    nsteps = 37
+    ours = pkg.host.process_profiling(times, nsteps)
+    ref = _report_rows(pyoracle.ref_profiling(times, nsteps))
+    assert len(ref) == 5
+    for k in range(5):
+        assert [f"{v:e}" for v in ours[k]] == [f"{v:e}" for v in ref[k]], (k, ours[k], ref[k])
+    if nattempts <= 2:
+        assert (ours[1, :3] == 0.0).all() and ours[1, 3] > 0.0
