@@ -158,6 +158,22 @@ int swe_hip_rk_substep(swe_hip_ctx *ctx, int substep_id);
  * Same arithmetic per node as swe_hip_rk_substep. */
 int swe_hip_rk_steps(swe_hip_ctx *ctx, int nsteps);
 
+/* No-allocation variant of swe_hip_rk_steps.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Advances the caller-owned
+ * device state H (n_nodes*4 doubles in the context's device layout:
+ * [node][u v w h] in the device node order, e.g. a copy of
+ * swe_hip_state_device_ptr after swe_hip_set_state) by nsteps fused RK4
+ * steps on the context's stream.  Same kernels and arithmetic as
+ * swe_hip_rk_steps, launched without a graph; the stage states and the RK
+ * sum live in a library-owned 64 MiB device array (one per GPU, held until
+ * the process ends) instead of context buffers, so the call makes no device
+ * allocation.  Limits: single-state contexts (members == 1, not tiled) of
+ * at most 699050 nodes (larger meshes fail with SWE_HIP_EINVAL).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * threads, other streams) race. */
+int swe_hip_rk_steps_noalloc(swe_hip_ctx *ctx, double *H, int nsteps);
+
 /* Test hook: one eval_RHS_swe (rk4_rbffd_swe.c:189-362) of a host state
  * K into F (host layouts), on member slot `member`. */
 int swe_hip_eval_rhs(swe_hip_ctx *ctx, int member, const double *K, double *F);

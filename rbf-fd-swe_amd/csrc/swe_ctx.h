@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <vector>
 
 namespace swe {
@@ -122,6 +123,14 @@ inline DevMesh mesh_of(const swe_hip_ctx *c)
     m.gh0 = c->gh0;
     return m;
 }
+
+// Records msg as the calling thread's last error (swe_hip_last_error) and
+// returns code.
+int set_error(int code, const std::string &msg);
+
+// Looks up this device's no-allocation scratch address (swe_noalloc.hip);
+// the device must be current.  On failure *err says why.
+int noalloc_setup(int dev, std::string *err);
 
 // One fused stage (ST_0..ST_3) or a plain RHS (ST_RHS) on the context's
 // stream, by kernel family (each in its own translation unit).

@@ -95,6 +95,12 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
+}  // namespace
+
+int swe::set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIPCHK(expr)                                                                   \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
@@ -698,6 +704,12 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "device ordinal out of range");
     if (hipSetDevice(device) != hipSuccess)
         return fail(SWE_HIP_EDEVICE, "hipSetDevice failed");
+    {
+        // device address of the no-allocation entry point's scratch, once
+        // per device; a failure is reported by that entry point only
+        std::string why;
+        (void)noalloc_setup(device, &why);
+    }
     SweMesh *m = nullptr;
     if (o->share) {
         m = o->share->mesh;

@@ -27,7 +27,7 @@ EXPORTS = (
     "swe_hip_strerror", "swe_hip_last_error", "swe_hip_create_ex", "swe_hip_query",
     "swe_hip_set_active", "swe_hip_move_member", "swe_hip_static_bytes",
     "swe_hip_tune_member_block",
-    "swe_hip_tune_ensemble",
+    "swe_hip_tune_ensemble", "swe_hip_rk_steps_noalloc",
 )
 
 
@@ -87,6 +87,7 @@ def lib() -> C.CDLL:
         L.swe_hip_get_state.argtypes = [vp, C.c_int, _dp]
         L.swe_hip_rk_substep.argtypes = [vp, C.c_int]
         L.swe_hip_rk_steps.argtypes = [vp, C.c_int]
+        L.swe_hip_rk_steps_noalloc.argtypes = [vp, vp, C.c_int]
         L.swe_hip_eval_rhs.argtypes = [vp, C.c_int, _dp, _dp]
         L.swe_hip_set_timing.argtypes = [vp, C.c_int]
         L.swe_hip_get_timers.argtypes = [vp, C.POINTER(SweHipTimers), C.c_int]
@@ -189,6 +190,12 @@ class HipContext:
 
     def rk_steps(self, nsteps: int) -> None:
         _chk(lib().swe_hip_rk_steps(self.handle, int(nsteps)), "swe_hip_rk_steps")
+
+    def rk_steps_noalloc(self, H_dev: int, nsteps: int) -> None:
+        """nsteps fused RK4 steps of the device state at address H_dev
+        (this context's device layout) with no device allocation."""
+        _chk(lib().swe_hip_rk_steps_noalloc(self.handle, C.c_void_p(H_dev), int(nsteps)),
+             "swe_hip_rk_steps_noalloc")
 
     def eval_rhs(self, K: np.ndarray, member: int = 0) -> np.ndarray:
         K = np.ascontiguousarray(K, dtype=np.float64)
