@@ -55,13 +55,15 @@ def parse_args():
     ap.add_argument("--nodes", type=int, default=40962)
     ap.add_argument("--members", type=int, default=1)
     ap.add_argument("--member-block", type=int, default=0)
-    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw", "pair"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw", "pair", "tile"])
     ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--order", default="hilbert", choices=["host", "morton", "hilbert"],
                     help="node order of the device copy (results are identical in any order; "
                          "Hilbert is the fastest measured at N=40962)")
     ap.add_argument("--tiled", action="store_true")
     ap.add_argument("--slot-block", type=int, default=0, choices=[0, 16])
+    ap.add_argument("--tile-nodes", type=int, default=0)
+    ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--layout", default="cfdl")
     ap.add_argument("--exact", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -129,7 +131,8 @@ def dominant_kernel(args):
         if args.kernel == "auto":
             return "k_stage_pair"
         return {"quadw": "k_stage_quadw", "quad": "k_stage_quad",
-                "lane": "k_stage_lane", "pair": "k_stage_pair"}[args.kernel]
+                "lane": "k_stage_lane", "pair": "k_stage_pair",
+                "tile": "k_stage_ptile"}[args.kernel]
     if args.tiled:
         return "k_stage_tspec" if args.member_block == 32 else "k_stage_tiled"
     return "k_stage_mpair" if args.member_block in (0, 64) and args.members % 64 == 0 \
@@ -270,7 +273,8 @@ def main():
     ctx = pkg.runtime.HipContext(prob, members=args.members, exact=args.exact,
                                  member_block=args.member_block, kernel=args.kernel,
                                  prefetch=args.prefetch, order=args.order, tiled=args.tiled,
-                                 slot_block=args.slot_block)
+                                 slot_block=args.slot_block, tile_nodes=args.tile_nodes,
+                                 tile_rows=args.tile_rows)
     for m in range(args.members):
         ctx.set_state(prob.H_init, m)
     stream = torch.cuda.ExternalStream(ctx.stream())

@@ -98,6 +98,11 @@ void swe_hip_destroy(swe_hip_ctx *ctx);
 #define SWE_HIP_KERNEL_PAIR 4   /* single state: 2 lanes per node, 16-byte
                                    weight and state loads (31-point stencils;
                                    other widths use QUADW)                   */
+#define SWE_HIP_KERNEL_TILE 5   /* single state: the PAIR mapping over tiles
+                                   of 128 consecutive device nodes whose
+                                   distinct neighbour states are staged in
+                                   LDS (31-point stencils, else QUADW; use a
+                                   HILBERT node order for compact tiles)     */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 #define SWE_HIP_ORDER_HILBERT 2 /* device node order = 3-D Hilbert curve     */
@@ -122,6 +127,12 @@ typedef struct swe_hip_options {
     int slot_block;             /* stencil arrays: 0 slot-major [nnbr][n];
                                    16 node blocks [n/16][nnbr][16] (ignored
                                    with share)                              */
+    int tile_nodes;             /* tile shape (0 = default): nodes per tile of
+                                   SWE_HIP_KERNEL_TILE (64, 128 or 256; up to
+                                   4x as many distinct neighbour rows), or of
+                                   the member-lane tiled kernel (1..8)      */
+    int tile_rows;              /* distinct rows per member-lane tile (0 =
+                                   70; 16..70)                              */
 } swe_hip_options;
 int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip_ctx **out);
 
@@ -168,7 +179,7 @@ int swe_hip_rk_steps(swe_hip_ctx *ctx, int nsteps);
  * swe_hip_rk_steps, launched without a graph; the stage states and the RK
  * sum live in a library-owned 64 MiB device array (one per GPU, held until
  * the process ends) instead of context buffers, so the call makes no device
- * allocation.  Limits: single-state contexts (members == 1, not tiled) of
+ * allocation.  Limits: single-state contexts (members == 1) of
  * at most 699050 nodes (larger meshes fail with SWE_HIP_EINVAL).  Every
  * call on one GPU shares that array: calls that overlap on one GPU (other
  * threads, other streams) race. */

@@ -66,9 +66,16 @@ int swe_roundup(const swe_layout *l, int n);
  * Dx, Dy, Dz by 1/a and L by gamma (input.c:314-319) and converts idx to
  * 0-based (input.c:326-328). */
 int swe_get_gsmd_bin(const char *path, swe_gsmd *g);
+/* The same for the FP32 model type (fp32 != 0, the reference's RUN_32BIT
+ * build): every value read is rounded to float, the weights once more after
+ * the double-precision scaling (input.c:211-319); values stay stored as
+ * doubles (each exactly a float). */
+int swe_get_gsmd_bin_ex(const char *path, int fp32, swe_gsmd *g);
 /* get_ICs_bin / get_FCs_bin: H_init and H_100, [Nnodes][4] (u,v,w,h). */
 int swe_get_ics_bin(const char *path, double **H_init);
 int swe_get_fcs_bin(const char *path, double **H_100);
+/* which 0 = H_init, 1 = H_100; fp32 rounds to float as above. */
+int swe_get_state_bin_ex(const char *path, int which, int fp32, double **H);
 void swe_gsmd_free(swe_gsmd *g);
 
 /* ---- RCM (swe_code/rcm/rcm.c:9-173, reorder_nodes.c:8-87) ---------------- */
@@ -104,6 +111,9 @@ typedef struct swe_problem {
 } swe_problem;
 int swe_problem_load(const char *path, int use_rcm, const swe_layout *l,
                      swe_problem *p);
+/* With fp32 != 0: the FP32 model type's reading (swe_get_gsmd_bin_ex). */
+int swe_problem_load_ex(const char *path, int use_rcm, const swe_layout *l, int fp32,
+                        swe_problem *p);
 void swe_problem_free(swe_problem *p);
 
 /* ---- verify_output (main.c:439-477) --------------------------------------- */

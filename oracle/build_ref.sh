@@ -10,6 +10,8 @@
 # Outputs (git-ignored, shipped with the tree):
 #   _ref/libswe_ref_<layout>.so  reference sources + ref_harness.c, strict
 #                                FP64 (-O2 -ffp-contract=off), per layout
+#   _ref/libswe_ref32_<layout>.so  the same with the reference's header as
+#                                shipped (RUN_32BIT: fType = float)
 #   _ref/swe_ref_<layout>.exe    the reference main() built like its best CPU
 #                                configuration (-O3 -fopenmp -DRHS_INNER_SIMD)
 # Layouts: default, cfdl, sfdl, cfdl_sfdl (compile_default.sh:3-6 matrix);
@@ -35,6 +37,7 @@ trap 'rm -rf "$TMP"' EXIT
 sed 's|^#define RUN_32BIT|// RUN_32BIT off: FP64 build|' "$S/main/include/swe_config.h" > "$TMP/swe_config.h"
 mkdir -p "$OUT"
 INC="-I$TMP -I$S/main/include -I$S/io/include -I$S/rcm/include -I$S/layout/include -I$S/mpi/include -I$S/ocl/include"
+INC32="-I$S/main/include -I$S/io/include -I$S/rcm/include -I$S/layout/include -I$S/mpi/include -I$S/ocl/include"
 LIBSRC="$S/main/rk4_rbffd_swe.c $S/main/runtime_params.c $S/main/profiling.c $S/io/input.c $S/rcm/rcm.c $S/rcm/reorder_nodes.c $S/layout/layout.c $S/layout/matrix_transformations.c $S/mpi/init_patches.c $S/mpi/halos.c"
 COMMON="-std=gnu99 -include time.h -DUSE_BINIO -DSIMD_LENGTH=$SIMD -DPADDED_NUM=$SIMD -w"
 for layout in default cfdl sfdl cfdl_sfdl; do
@@ -46,6 +49,8 @@ for layout in default cfdl sfdl cfdl_sfdl; do
     esac
     gcc $COMMON $LF -O2 -ffp-contract=off -fopenmp -fPIC -shared $INC \
         "$HERE/ref_harness.c" $LIBSRC -lm -o "$OUT/libswe_ref_$layout.so"
+    gcc $COMMON $LF -O2 -ffp-contract=off -fopenmp -fPIC -shared $INC32 \
+        "$HERE/ref_harness.c" $LIBSRC -lm -o "$OUT/libswe_ref32_$layout.so"
     gcc $COMMON $LF -O3 $ARCH -fopenmp -DRHS_INNER_SIMD $INC \
         "$S/main/main.c" $LIBSRC -lm -o "$OUT/swe_ref_$layout.exe"
 done

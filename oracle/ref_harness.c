@@ -3,7 +3,8 @@
  *
  * Built by oracle/build_ref.sh together with the reference's source files,
  * compiled where they lie under the reference tree, into oracle/_ref/.  It
- * replaces only the reference's main() (swe_code/main/main.c:45): it owns the
+ * replaces only the reference's main() (swe_code/main/main.c:45), for either
+ * model precision (the header's fType), and it owns the
  * globals main.c defines (main.c:34-41) and runs the same serial
  * initialisation sequence (main.c:94-190), then exposes the reference's
  * RK_substep / eval_RHS_swe and the prepared arrays to the tests through a
@@ -37,11 +38,13 @@ static fType *g_H_init, *g_H_100;
 static int *g_mapping;
 static int g_Nnodes, g_padded_Nnodes;
 
+/* sizeof(fType): 8 for the FP64 header variant, 4 for the reference's
+ * shipped RUN_32BIT header; every array below is of fType. */
+int ref_ftype_size(void) { return (int)sizeof(fType); }
+
 /* main.c:94-190 for mpi_size == 1 and binary input. Returns 0 on success. */
 int ref_setup(const char *path, int use_rcm, int nattempts)
 {
-    if (sizeof(fType) != sizeof(double))
-        return -1; /* the harness is only built against the FP64 header */
     memset(&sim_params, 0, sizeof(sim_params));
     strncpy(sim_params.inputFile, path, MAX_FILE_PATH_SIZE - 1);
     sim_params.nattempts = nattempts;
@@ -108,13 +111,13 @@ void ref_info(int *out)
 
 void ref_scalars(double *out)
 {
-    out[0] = LPSMD->gh0;
-    out[1] = LPSMD->dt;
+    out[0] = (double)LPSMD->gh0;
+    out[1] = (double)LPSMD->dt;
 }
 
 /* Pointers to the 13 prepared arrays, in the order
  * x y z f ghm p_u p_v p_w gradghm Dx Dy Dz L, then idx separately. */
-void ref_arrays(const double **fp, const int **idx)
+void ref_arrays(const fType **fp, const int **idx)
 {
     fp[0] = LPSMD->x;   fp[1] = LPSMD->y;   fp[2] = LPSMD->z;
     fp[3] = LPSMD->f;   fp[4] = LPSMD->ghm;
@@ -125,17 +128,17 @@ void ref_arrays(const double **fp, const int **idx)
     *idx = LPSMD->idx;
 }
 
-const double *ref_H_init(void) { return g_H_init; }
-const double *ref_H_100(void) { return g_H_100; }
+const fType *ref_H_init(void) { return g_H_init; }
+const fType *ref_H_100(void) { return g_H_100; }
 const int *ref_mapping(void) { return g_mapping; }
 
-void ref_rk_substep(double *H, double *K, double *F, double *D, int s)
+void ref_rk_substep(fType *H, fType *K, fType *F, fType *D, int s)
 {
     local_timer.attempt = 0;
     RK_substep(H, K, F, D, s);
 }
 
-void ref_eval_rhs(double *K, double *F)
+void ref_eval_rhs(fType *K, fType *F)
 {
     local_timer.attempt = 0;
     eval_RHS_swe(K, F);

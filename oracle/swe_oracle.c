@@ -13,11 +13,18 @@
  *   verify_output  swe_code/main/main.c:439-477
  *   index macros   swe_code/main/include/swe_config.h:21-54
  *
- * Two flavours are built from this file:
+ * Three flavours are built from this file:
  *   strict  (-O2 -ffp-contract=off): sequential j, no FMA -> the checker;
  *   fast    (-O3 -march=... -DORACLE_INNER_SIMD): the reference's best CPU
  *           configuration (omp simd reduction over j, rk4_rbffd_swe.c:275-278),
- *           used only as the timed CPU baseline.
+ *           used only as the timed CPU baseline;
+ *   strict32 (-DORACLE_FP32): the model type is float, as in the reference's
+ *           shipped RUN_32BIT build (swe_config.h:75-80).  Every expression
+ *           keeps the reference's C types, so its promotions carry over: the
+ *           coefficients dt/2.0 (eval_K) are computed in double and rounded
+ *           to float as fType arguments, while update_H's (dt/6.0)*D and the
+ *           sum into H are double, rounded to float on the store
+ *           (rk4_rbffd_swe.c:86-121).
  */
 #include "swe_oracle.h"
 
@@ -68,38 +75,38 @@ static inline long svm_lin(int pid, int sv, int cfdl, int S)
 #define ORACLE_RHS_BODY(CF, SF)                                                 \
     const int S = m->simd, nnbr = m->Nnbr, pnnbr = m->padded_Nnbr;              \
     const int cps = m->compute_pid_s, n = m->compute_size;                      \
-    const double gh0 = m->gh0;                                                  \
+    const oracle_real gh0 = m->gh0;                                             \
     _Pragma("omp parallel for schedule(static)")                               \
     for (int i = 0; i < n; i++) {                                               \
-        double dudx = 0.0, dvdx = 0.0, dwdx = 0.0, dhdx = 0.0;                  \
-        double dudy = 0.0, dvdy = 0.0, dwdy = 0.0, dhdy = 0.0;                  \
-        double dudz = 0.0, dvdz = 0.0, dwdz = 0.0, dhdz = 0.0;                  \
-        double lu = 0.0, lv = 0.0, lw = 0.0, lh = 0.0;                          \
+        oracle_real dudx = 0.0, dvdx = 0.0, dwdx = 0.0, dhdx = 0.0;                  \
+        oracle_real dudy = 0.0, dvdy = 0.0, dwdy = 0.0, dhdy = 0.0;                  \
+        oracle_real dudz = 0.0, dvdz = 0.0, dwdz = 0.0, dhdz = 0.0;                  \
+        oracle_real lu = 0.0, lv = 0.0, lw = 0.0, lh = 0.0;                          \
         ORACLE_J_PRAGMA                                                         \
         for (int j = 0; j < nnbr; j++) {                                        \
             const long d = dm_lin(i, j, pnnbr, SF, S);                          \
             const int q = m->idx[d];                                            \
-            const double su = K[svm_lin(q, 0, CF, S)];                          \
-            const double sv = K[svm_lin(q, 1, CF, S)];                          \
-            const double sw = K[svm_lin(q, 2, CF, S)];                          \
-            const double sh = K[svm_lin(q, 3, CF, S)];                          \
-            const double wx = m->Dx[d], wy = m->Dy[d], wz = m->Dz[d];           \
-            const double wl = m->L[d];                                          \
+            const oracle_real su =K[svm_lin(q, 0, CF, S)];                          \
+            const oracle_real sv =K[svm_lin(q, 1, CF, S)];                          \
+            const oracle_real sw =K[svm_lin(q, 2, CF, S)];                          \
+            const oracle_real sh =K[svm_lin(q, 3, CF, S)];                          \
+            const oracle_real wx = m->Dx[d], wy = m->Dy[d], wz = m->Dz[d]; \
+            const oracle_real wl = m->L[d];                                     \
             dudx += wx * su; dvdx += wx * sv; dwdx += wx * sw; dhdx += wx * sh; \
             dudy += wy * su; dvdy += wy * sv; dwdy += wy * sw; dhdy += wy * sh; \
             dudz += wz * su; dvdz += wz * sv; dwdz += wz * sw; dhdz += wz * sh; \
             lu += wl * su;   lv += wl * sv;   lw += wl * sw;   lh += wl * sh;   \
         }                                                                       \
-        const double u = K[svm_lin(i + cps, 0, CF, S)];                         \
-        const double v = K[svm_lin(i + cps, 1, CF, S)];                         \
-        const double w = K[svm_lin(i + cps, 2, CF, S)];                         \
-        const double h = K[svm_lin(i + cps, 3, CF, S)];                         \
-        const double fi = m->f[i], xi = m->x[i], yi = m->y[i], zi = m->z[i];    \
-        const double ru = -((u * dudx) + (v * dudy) + (w * dudz)                \
+        const oracle_real u =K[svm_lin(i + cps, 0, CF, S)];                         \
+        const oracle_real v =K[svm_lin(i + cps, 1, CF, S)];                         \
+        const oracle_real w =K[svm_lin(i + cps, 2, CF, S)];                         \
+        const oracle_real h =K[svm_lin(i + cps, 3, CF, S)];                         \
+        const oracle_real fi =m->f[i], xi = m->x[i], yi = m->y[i], zi = m->z[i];    \
+        const oracle_real ru =-((u * dudx) + (v * dudy) + (w * dudz)                \
                             + (fi * ((yi * w) - (zi * v))) + dhdx);             \
-        const double rv = -((u * dvdx) + (v * dvdy) + (w * dvdz)                \
+        const oracle_real rv =-((u * dvdx) + (v * dvdy) + (w * dvdz)                \
                             + (fi * ((zi * u) - (xi * w))) + dhdy);             \
-        const double rw = -((u * dwdx) + (v * dwdy) + (w * dwdz)                \
+        const oracle_real rw =-((u * dwdx) + (v * dwdy) + (w * dwdz)                \
                             + (fi * ((xi * v) - (yi * u))) + dhdz);             \
         const long cx = vect_lin(i, 0, SF, S), cy = vect_lin(i, 1, SF, S);      \
         const long cz = vect_lin(i, 2, SF, S);                                  \
@@ -116,12 +123,12 @@ static inline long svm_lin(int pid, int sv, int cfdl, int S)
             + lh;                                                               \
     }
 
-static void rhs_00(const oracle_psmd *m, const double *K, double *F) { ORACLE_RHS_BODY(0, 0) }
-static void rhs_10(const oracle_psmd *m, const double *K, double *F) { ORACLE_RHS_BODY(1, 0) }
-static void rhs_01(const oracle_psmd *m, const double *K, double *F) { ORACLE_RHS_BODY(0, 1) }
-static void rhs_11(const oracle_psmd *m, const double *K, double *F) { ORACLE_RHS_BODY(1, 1) }
+static void rhs_00(const oracle_psmd *m, const oracle_real *K, oracle_real *F) { ORACLE_RHS_BODY(0, 0) }
+static void rhs_10(const oracle_psmd *m, const oracle_real *K, oracle_real *F) { ORACLE_RHS_BODY(1, 0) }
+static void rhs_01(const oracle_psmd *m, const oracle_real *K, oracle_real *F) { ORACLE_RHS_BODY(0, 1) }
+static void rhs_11(const oracle_psmd *m, const oracle_real *K, oracle_real *F) { ORACLE_RHS_BODY(1, 1) }
 
-void oracle_eval_rhs(const oracle_psmd *m, const double *K, double *F,
+void oracle_eval_rhs(const oracle_psmd *m, const oracle_real *K, oracle_real *F,
                      oracle_timers *t)
 {
     const double t0 = now_s();
@@ -138,8 +145,8 @@ void oracle_eval_rhs(const oracle_psmd *m, const double *K, double *F,
 }
 
 /* eval_K, rk4_rbffd_swe.c:123-158: K = H + c*F over the compute range. */
-static void eval_k(const oracle_psmd *m, const double *H, double *K,
-                   const double *F, double c, oracle_timers *t)
+static void eval_k(const oracle_psmd *m, const oracle_real *H, oracle_real *K,
+                   const oracle_real *F, oracle_real c, oracle_timers *t)
 {
     const double t0 = now_s();
     const long off = 4L * m->compute_pid_s, n4 = 4L * m->compute_size;
@@ -151,8 +158,8 @@ static void eval_k(const oracle_psmd *m, const double *H, double *K,
 }
 
 /* update_D, rk4_rbffd_swe.c:160-186: D += coeff*F. */
-static void update_d(const oracle_psmd *m, double *D, const double *F,
-                     double coeff, oracle_timers *t)
+static void update_d(const oracle_psmd *m, oracle_real *D, const oracle_real *F,
+                     oracle_real coeff, oracle_timers *t)
 {
     const double t0 = now_s();
     const long n4 = 4L * m->compute_size;
@@ -164,8 +171,8 @@ static void update_d(const oracle_psmd *m, double *D, const double *F,
 }
 
 /* update_H, rk4_rbffd_swe.c:86-121: H += (dt/6)*D. */
-static void update_h(const oracle_psmd *m, double *H, const double *D,
-                     double dt, oracle_timers *t)
+static void update_h(const oracle_psmd *m, oracle_real *H, const oracle_real *D,
+                     oracle_real dt, oracle_timers *t)
 {
     const double t0 = now_s();
     const long off = 4L * m->compute_pid_s, n4 = 4L * m->compute_size;
@@ -176,14 +183,14 @@ static void update_h(const oracle_psmd *m, double *H, const double *D,
         t->t_update_H += now_s() - t0;
 }
 
-void oracle_rk_substep(const oracle_psmd *m, double *H, double *K, double *F,
-                       double *D, int substep_id, oracle_timers *t)
+void oracle_rk_substep(const oracle_psmd *m, oracle_real *H, oracle_real *K, oracle_real *F,
+                       oracle_real *D, int substep_id, oracle_timers *t)
 {
-    const double dt = m->dt;
+    const oracle_real dt = m->dt;
     switch (substep_id) {
     case 0:
         oracle_eval_rhs(m, H, F, t);
-        memcpy(D, F, sizeof(double) * 4L * m->compute_size);
+        memcpy(D, F, sizeof(oracle_real) * 4L * m->compute_size);
         eval_k(m, H, K, F, dt / 2.0, t);
         break;
     case 1:
@@ -206,15 +213,15 @@ void oracle_rk_substep(const oracle_psmd *m, double *H, double *K, double *F,
     }
 }
 
-void oracle_run(const oracle_psmd *m, double *H, double *K, double *F,
-                double *D, int nsteps, oracle_timers *t)
+void oracle_run(const oracle_psmd *m, oracle_real *H, oracle_real *K, oracle_real *F,
+                oracle_real *D, int nsteps, oracle_timers *t)
 {
     for (int step = 1; step <= nsteps; step++)
         for (int s = 0; s < 4; s++)
             oracle_rk_substep(m, H, K, F, D, s, t);
 }
 
-int oracle_verify(const oracle_psmd *m, const double *H1, const double *H2,
+int oracle_verify(const oracle_psmd *m, const oracle_real *H1, const oracle_real *H2,
                   double tol)
 {
     int count = 0;
@@ -222,7 +229,7 @@ int oracle_verify(const oracle_psmd *m, const double *H1, const double *H2,
     for (int i = lo; i < hi; i++) {
         for (int sv = 0; sv < 4; sv++) {
             const long k = svm_lin(i, sv, m->cfdl, m->simd);
-            double diff = H1[k] - H2[k];
+            oracle_real diff = H1[k] - H2[k];
             diff = diff < 0 ? -diff : diff;
             if (diff > tol || diff != diff)
                 count++;
@@ -248,3 +255,5 @@ int oracle_num_threads(void)
     return 1;
 #endif
 }
+
+int oracle_real_size(void) { return (int)sizeof(oracle_real); }

@@ -25,6 +25,15 @@
 extern "C" {
 #endif
 
+/* Model precision (the reference's fType, swe_config.h:70-84): double, or
+ * float in the FP32 build of this file (ORACLE_FP32, liboracle32.so), which
+ * follows the reference's RUN_32BIT build, C promotions included. */
+#ifdef ORACLE_FP32
+typedef float oracle_real;
+#else
+typedef double oracle_real;
+#endif
+
 /* Local patch static model data in the reference host layout
  * (swe_code/main/include/swe_config.h:187-233, serial single patch).
  * Index macros (swe_config.h:21-54) are evaluated with `cfdl`, `sfdl`
@@ -36,11 +45,11 @@ typedef struct oracle_psmd {
     int Nnbr;           /* real stencil width (loop bound, rk4:285)        */
     int padded_Nnbr;    /* row stride of idx/Dx/Dy/Dz/L                    */
     int cfdl, sfdl, simd;
-    double gh0, dt;
-    const double *x, *y, *z, *f, *ghm;
-    const double *p_u, *p_v, *p_w, *gradghm;
+    oracle_real gh0, dt;
+    const oracle_real *x, *y, *z, *f, *ghm;
+    const oracle_real *p_u, *p_v, *p_w, *gradghm;
     const int *idx;
-    const double *Dx, *Dy, *Dz, *L;
+    const oracle_real *Dx, *Dy, *Dz, *L;
 } oracle_psmd;
 
 /* Per-phase wall-clock accumulators, seconds (timing_struct,
@@ -50,28 +59,30 @@ typedef struct oracle_timers {
 } oracle_timers;
 
 /* eval_RHS_swe (rk4_rbffd_swe.c:189-362): F = RHS(K). */
-void oracle_eval_rhs(const oracle_psmd *m, const double *K, double *F,
+void oracle_eval_rhs(const oracle_psmd *m, const oracle_real *K, oracle_real *F,
                      oracle_timers *t);
 
 /* RK_substep (rk4_rbffd_swe.c:30-72) with eval_K/update_D/update_H/
  * copy_fp_arr (:74-186). */
-void oracle_rk_substep(const oracle_psmd *m, double *H, double *K, double *F,
-                       double *D, int substep_id, oracle_timers *t);
+void oracle_rk_substep(const oracle_psmd *m, oracle_real *H, oracle_real *K, oracle_real *F,
+                       oracle_real *D, int substep_id, oracle_timers *t);
 
 /* The time loop of main.c:336-358 for one attempt: H advanced nsteps.
  * Scratch K (patch_size*4), F and D (compute_size*4) are caller-owned. */
-void oracle_run(const oracle_psmd *m, double *H, double *K, double *F,
-                double *D, int nsteps, oracle_timers *t);
+void oracle_run(const oracle_psmd *m, oracle_real *H, oracle_real *K, oracle_real *F,
+                oracle_real *D, int nsteps, oracle_timers *t);
 
 /* verify_output (main.c:439-477): number of entries over the compute range
  * whose |H1-H2| exceeds tol, or that are NaN. */
-int oracle_verify(const oracle_psmd *m, const double *H1, const double *H2,
+int oracle_verify(const oracle_psmd *m, const oracle_real *H1, const oracle_real *H2,
                   double tol);
 
 /* 1 if built with the reference's inner-SIMD reduction (RHS_INNER_SIMD),
  * i.e. the fast timed flavour; 0 for the strict sequential flavour. */
 int oracle_flavour_inner_simd(void);
 int oracle_num_threads(void);
+/* sizeof(oracle_real): 8, or 4 in the FP32 build. */
+int oracle_real_size(void);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,38 @@ void launch_pair(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
     }
 }
 
+// Pair mapping over LDS-staged tiles: one workgroup of 2 x tile_nodes lanes
+// per tile; prefetch selects the row staging (0 registers, 4 direct-to-LDS,
+// 8 direct-to-LDS + early weights).
+template <int STAGE, bool EX, int NN, int TN>
+void launch_ptile_tn(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    const DevTiles tl = tiles_of(c, 1);
+    st.nblk = (tl.ntiles + 7) / 8 * 8;
+    const dim3 g(st.nblk), b(2 * TN);
+    if (c->prefetch == 8)
+        k_stage_ptile<STAGE, EX, NN, TN, 2><<<g, b, 0, c->stream>>>(m, tl, st);
+    else if (c->prefetch == 4)
+        k_stage_ptile<STAGE, EX, NN, TN, 1><<<g, b, 0, c->stream>>>(m, tl, st);
+    else
+        k_stage_ptile<STAGE, EX, NN, TN, 0><<<g, b, 0, c->stream>>>(m, tl, st);
+}
+
+template <int STAGE, bool EX, int NN>
+void launch_ptile(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
+{
+    if constexpr (NN > 0) {
+        if (c->tile_nodes == 64)
+            launch_ptile_tn<STAGE, EX, NN, 64>(c, m, st);
+        else if (c->tile_nodes == 256)
+            launch_ptile_tn<STAGE, EX, NN, 256>(c, m, st);
+        else
+            launch_ptile_tn<STAGE, EX, NN, 128>(c, m, st);
+    } else {
+        launch_quadw<STAGE, EX, NN>(c, m, st);   // needs a compile-time stencil width
+    }
+}
+
 }  // namespace
 
 template <int STAGE, bool EX, int NN>
@@ -66,6 +98,8 @@ void family_single(const swe_hip_ctx *c, const DevStage &st)
         launch_quadw<STAGE, EX, NN>(c, m, st);
     else if (c->kernel == SWE_HIP_KERNEL_PAIR)
         launch_pair<STAGE, EX, NN>(c, m, st);
+    else if (c->kernel == SWE_HIP_KERNEL_TILE && c->tiles.ntiles > 0)
+        launch_ptile<STAGE, EX, NN>(c, m, st);
     else
         launch_quad<STAGE, EX, NN>(c, m, st);
 }

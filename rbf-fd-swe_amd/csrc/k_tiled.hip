@@ -4,22 +4,6 @@
 namespace swe {
 namespace {
 
-DevTiles tiles_of(const swe_hip_ctx *c, int mb)
-{
-    DevTiles tl;
-    tl.tile_start = c->tiles.d_tile_start;
-    tl.ustart = c->tiles.d_ustart;
-    tl.ulist = c->tiles.d_ulist;
-    tl.lidx = c->tiles.d_lidx;
-    tl.lidx_node = c->tiles.d_lidx_node;
-    tl.ulist_tile = c->tiles.d_ulist_tile;
-    tl.wts_node = c->tiles.d_wts_node;
-    tl.geo_node = c->tiles.d_geo_node;
-    tl.ntiles = c->tiles.ntiles;
-    tl.nmb = c->B / mb;
-    return tl;
-}
-
 template <int STAGE, bool EX, int MBv, int NN>
 void launch_tiled(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
 {

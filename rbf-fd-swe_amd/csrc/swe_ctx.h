@@ -84,6 +84,7 @@ struct swe_hip_ctx {
     int B = 1, MB = 1;
     int kernel = SWE_HIP_KERNEL_QUAD;   // single-state mapping
     int prefetch = 0;                   // stencil slots per load batch (0, 4, 8)
+    int tile_nodes = 0, tile_rows = 0;  // tile shape of the tiled kernels (0: default)
     int ncu = 0;                        // compute units of the device
     unsigned flags = 0;
     double *d_H = nullptr, *d_Ka = nullptr, *d_Kb = nullptr, *d_D = nullptr, *d_F = nullptr;
@@ -140,12 +141,29 @@ void launch_tiled_stage(const swe_hip_ctx *c, int stage, const DevStage &st);
 
 inline void launch_stage(const swe_hip_ctx *c, int stage, const DevStage &st)
 {
-    if (c->tiles.ntiles > 0)
-        launch_tiled_stage(c, stage, st);
-    else if (c->B == 1)
+    if (c->B == 1)
         launch_single_stage(c, stage, st);
+    else if (c->tiles.ntiles > 0)
+        launch_tiled_stage(c, stage, st);
     else
         launch_members_stage(c, stage, st);
+}
+
+// Device view of the context's tiles for member block mb.
+inline DevTiles tiles_of(const swe_hip_ctx *c, int mb)
+{
+    DevTiles tl;
+    tl.tile_start = c->tiles.d_tile_start;
+    tl.ustart = c->tiles.d_ustart;
+    tl.ulist = c->tiles.d_ulist;
+    tl.lidx = c->tiles.d_lidx;
+    tl.lidx_node = c->tiles.d_lidx_node;
+    tl.ulist_tile = c->tiles.d_ulist_tile;
+    tl.wts_node = c->tiles.d_wts_node;
+    tl.geo_node = c->tiles.d_geo_node;
+    tl.ntiles = c->tiles.ntiles;
+    tl.nmb = c->B / mb;
+    return tl;
 }
 
 }  // namespace swe

@@ -588,7 +588,7 @@ namespace {
 int time_candidate(swe_hip_ctx *base, int members, unsigned flags, int mb, int tiled, int steps,
                    double *per_step)
 {
-    swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, tiled, 0};
+    swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, tiled, 0, 0, 0};
     swe_hip_ctx *c = nullptr;
     int rc = swe_hip_create_ex(nullptr, &o, &c);
     if (rc)
@@ -675,7 +675,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "static data is NULL");
     if (o->members < 1 || o->members > 4096)
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
-    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_PAIR)
+    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_TILE)
         return fail(SWE_HIP_EINVAL, "kernel must be one of SWE_HIP_KERNEL_*");
     if (o->tiled < 0 || o->tiled > 1)
         return fail(SWE_HIP_EINVAL, "tiled must be 0 or 1");
@@ -685,6 +685,14 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "node_order must be SWE_HIP_ORDER_HOST, _MORTON or _HILBERT");
     if (o->slot_block != 0 && o->slot_block != 16)
         return fail(SWE_HIP_EINVAL, "slot_block must be 0 or 16");
+    if (o->members == 1 && o->tile_nodes != 0 && o->tile_nodes != 64 && o->tile_nodes != 128
+        && o->tile_nodes != 256)
+        return fail(SWE_HIP_EINVAL, "tile_nodes of the single-state tile kernel must be 0, 64, "
+                                    "128 or 256");
+    if (o->members > 1 && (o->tile_nodes < 0 || o->tile_nodes > kTmemTile))
+        return fail(SWE_HIP_EINVAL, "tile_nodes of the member-lane tiled kernel must be 0..8");
+    if (o->tile_rows != 0 && (o->tile_rows < 16 || o->tile_rows > kTmemRows))
+        return fail(SWE_HIP_EINVAL, "tile_rows must be 0 or 16..70");
     if (!o->share) {
         const int rc = validate_static(s);
         if (rc)
@@ -739,9 +747,14 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     // N = 40962, 163842 and 655362; it needs a compile-time stencil width.
     if (o->kernel == SWE_HIP_KERNEL_AUTO)
         c->kernel = m->nnbr == 31 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_QUADW;
+    else if ((o->kernel == SWE_HIP_KERNEL_PAIR || o->kernel == SWE_HIP_KERNEL_TILE)
+             && m->nnbr != 31)
+        c->kernel = SWE_HIP_KERNEL_QUADW;   // both need a compile-time stencil width
     else
         c->kernel = o->kernel;
     c->prefetch = o->prefetch;
+    c->tile_nodes = o->tile_nodes;
+    c->tile_rows = o->tile_rows;
     c->flags = o->flags;
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess || c->ncu <= 0)
@@ -767,6 +780,11 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         k_set_int<<<1, 64, 0, c->stream>>>(c->d_active, c->B);
         rc = check_launch("k_set_int");
     }
+    if (!rc && c->B == 1 && c->kernel == SWE_HIP_KERNEL_TILE) {
+        if (!c->tile_nodes)
+            c->tile_nodes = kPtileNodes;
+        rc = build_tiles(c, c->tile_nodes, 4 * c->tile_nodes);
+    }
     if (!rc && o->tiled) {
         int lds_max = 0;
         if (c->MB == 32 && c->B >= 32) {
@@ -779,7 +797,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
                 rc = fail(SWE_HIP_EDEVICE, "the member-lane tiled kernel needs 160 KB of LDS "
                                            "per workgroup");
             else
-                rc = build_tiles(c, kTmemTile, kTmemRows);
+                rc = build_tiles(c, c->tile_nodes ? c->tile_nodes : kTmemTile,
+                                 c->tile_rows ? c->tile_rows : kTmemRows);
         } else if (c->B < 2 || (c->MB != 4 && c->MB != 8 && c->MB != 16)) {
             rc = fail(SWE_HIP_EINVAL, "the tiled ensemble kernel needs members >= 2 and a "
                                       "member block of 4, 8, 16 or 32");
@@ -800,7 +819,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
 int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int device,
                    swe_hip_ctx **out)
 {
-    swe_hip_options o{members, flags, device, 0, nullptr, 0, 0, 0, 0, 0};
+    swe_hip_options o{members, flags, device, 0, nullptr, 0, 0, 0, 0, 0, 0, 0};
     return swe_hip_create_ex(s, &o, out);
 }
 

@@ -905,6 +905,146 @@ __global__ __launch_bounds__(512) void k_stage_tiled(DevMesh m, DevTiles tl, Dev
         stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
 }
 
+// 16 bytes per lane from global memory straight into LDS (gfx950
+// global_load_lds_dwordx4): lane l's bytes land at lds + 16 l, lds being
+// wave-uniform.  No VGPR holds the data and no LDS write instruction is
+// issued; the wave's vmcnt counts the load until the bytes are in LDS.
+__device__ __forceinline__ void dma16(const void *src, double *lds)
+{
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds, 16, 0,
+                                     0);
+}
+
+// ---------------------------------------------------------------------------
+// Single state (B == 1), pair mapping over LDS tiles.  The device mesh is in
+// a space-filling-curve order cut into tiles of <= TN consecutive nodes
+// whose stencils touch <= 4 TN distinct nodes (~3 per node at 128-node
+// Hilbert tiles, against 31 stencil entries per node).  A workgroup (2 TN
+// lanes) copies its tile's distinct neighbour states (32 B rows) into LDS,
+// then runs the pair mapping of k_stage_pair with every neighbour read from
+// LDS through the 16-bit tile positions (node-major, 64 B per node, two
+// 16-byte loads per lane): the 31 scattered global gathers per node, which
+// keep the texture path of k_stage_pair busy at large N, become LDS reads.
+// Own state: the node's row (position in the spare slot 31), else a global
+// load.  Same sums, same order as k_stage_pair (bitwise in exact mode).
+// LOAD selects the staging schedule: 0 rows through registers; 1 rows by
+// direct-to-LDS loads; 2 the same plus the weights of the first kPtilePre
+// slots loaded before the barrier that waits for the rows.
+constexpr int kPtileNodes = 128;   // default nodes per tile (4 waves x 32 nodes)
+constexpr int kPtilePre = 8;
+
+template <int TN>
+constexpr int ptile_rows() { return 4 * TN; }
+
+template <int STAGE, bool EXACT, int NNBR, int TN, int LOAD>
+__global__ __launch_bounds__(2 * TN) void k_stage_ptile(DevMesh m, DevTiles tl, DevStage st)
+{
+    static_assert(NNBR > 0 && NNBR <= 31, "positions: 31 slots + the own row");
+    static_assert(TN % 32 == 0, "whole waves");
+    __shared__ double2 rows[2 * ptile_rows<TN>()];   // [row][half] = (u, v), (w, h)
+    const int tile = xcd_block(st.nblk);
+    if (tile >= tl.ntiles)
+        return;   // uniform over the workgroup
+    const int t0 = tl.tile_start[tile];
+    const int tcount = tl.tile_start[tile + 1] - t0;
+    const int u0 = tl.ustart[tile];
+    const int ucount = tl.ustart[tile + 1] - u0;
+    const double *__restrict__ src = st.src;
+    if constexpr (LOAD == 0) {
+        for (int i = (int)threadIdx.x; i < 2 * ucount; i += (int)blockDim.x) {
+            const int u = tl.ulist[u0 + (i >> 1)];
+            rows[i] = *reinterpret_cast<const double2 *>(src + 4L * u + 2 * (i & 1));
+        }
+    } else {
+        // wave w copies rows [32 c, 32 c + 32) for c = w, w + waves, ...: lane
+        // l's 16 bytes are half (l & 1) of row 32 c + l / 2, landing at
+        // rows + 64 c + l (double2)
+        const int lane = (int)threadIdx.x & 63;
+        const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+        for (int c = wv; 32 * c < ucount; c += TN / 32) {
+            const int r = 32 * c + (lane >> 1);
+            const int u = tl.ulist[u0 + (r < ucount ? r : ucount - 1)];
+            if (r < ucount)
+                dma16(src + 4L * u + 2 * (lane & 1), reinterpret_cast<double *>(rows + 64 * c));
+        }
+    }
+    const int nl = (int)threadIdx.x >> 1;
+    const int k = (int)threadIdx.x & 1;
+    const bool live = nl < tcount;
+    const int node = t0 + (live ? nl : tcount - 1);   // dead pairs stay whole for DPP
+    const long ld = slot_stride(m);
+    const double *__restrict__ wp = m.wts + 4L * slot_base<NNBR>(m, node) + 2 * k;
+    // positions of slots 16k .. 16k + 15 (two per dword)
+    const uint4 *pp = reinterpret_cast<const uint4 *>(tl.lidx_node + 32L * node) + 2 * k;
+    const uint4 pa = pp[0], pb = pp[1];
+    const unsigned pw[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    constexpr int NPRE = LOAD == 2 ? kPtilePre : 0;
+    double2 wpre[NPRE > 0 ? NPRE : 1];
+    static_for<NPRE>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        wpre[j] = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+    });
+    if constexpr (LOAD != 0)
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's row loads are in LDS
+    __syncthreads();
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;   // variable 2k   x (own0, own1, oth0, oth1)
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;   // variable 2k+1 x (own0, own1, oth0, oth1)
+    static_for<NNBR>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const unsigned wv = (unsigned)pair_bcast_i<j / 16>((int)pw[(j % 16) / 2]);
+        const int pos = (int)((j & 1) ? (wv >> 16) : (wv & 0xffffu));
+        double2 w;
+        if constexpr (j < NPRE)
+            w = wpre[j];
+        else
+            w = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+        const double2 s = rows[2 * pos + k];
+        const double o0 = pair_swap(w.x), o1 = pair_swap(w.y);
+        a0 = madd<EXACT>(a0, w.x, s.x); a1 = madd<EXACT>(a1, w.y, s.x);
+        a2 = madd<EXACT>(a2, o0, s.x);  a3 = madd<EXACT>(a3, o1, s.x);
+        b0 = madd<EXACT>(b0, w.x, s.y); b1 = madd<EXACT>(b1, w.y, s.y);
+        b2 = madd<EXACT>(b2, o0, s.y);  b3 = madd<EXACT>(b3, o1, s.y);
+    });
+    const double ax = k ? a2 : a0, ay = k ? a3 : a1, az = k ? a0 : a2, al = k ? a1 : a3;
+    const double bx = k ? b2 : b0, by = k ? b3 : b1, bz = k ? b0 : b2, bl = k ? b1 : b3;
+    const double pax = pair_swap(ax), pay = pair_swap(ay), paz = pair_swap(az), pal = pair_swap(al);
+    const double pbx = pair_swap(bx), pby = pair_swap(by), pbz = pair_swap(bz), pbl = pair_swap(bl);
+    Sums S;
+    S.dudx = k ? pax : ax; S.dudy = k ? pay : ay; S.dudz = k ? paz : az; S.lu = k ? pal : al;
+    S.dvdx = k ? pbx : bx; S.dvdy = k ? pby : by; S.dvdz = k ? pbz : bz; S.lv = k ? pbl : bl;
+    S.dwdx = k ? ax : pax; S.dwdy = k ? ay : pay; S.dwdz = k ? az : paz; S.lw = k ? al : pal;
+    S.dhdx = k ? bx : pbx; S.dhdy = k ? by : pby; S.dhdz = k ? bz : pbz; S.lh = k ? bl : pbl;
+    // own row: position in the spare slot 31 (lane 1's last dword, high half)
+    const unsigned ownw = (unsigned)pair_bcast_i<1>((int)pw[7]) >> 16;
+    const double2 own = ownw != 0xffffu
+                            ? rows[2 * (int)ownw + k]
+                            : *reinterpret_cast<const double2 *>(src + 4L * node + 2 * k);
+    const double po0 = pair_swap(own.x), po1 = pair_swap(own.y);
+    const double u = k ? po0 : own.x, v = k ? po1 : own.y;
+    const double w = k ? own.x : po0, h = k ? own.y : po1;
+    double gv[(GEO_N + 1) / 2];
+#pragma unroll
+    for (int r = 0; r < (GEO_N + 1) / 2; ++r) {
+        const int f = 2 * r + k;
+        gv[r] = m.geo[(long)(f < GEO_N ? f : 0) * m.ld + node];
+    }
+    double gf[GEO_N];
+    static_for<GEO_N>([&](auto fc) {
+        constexpr int f = decltype(fc)::value;
+        gf[f] = pair_bcast<f & 1>(gv[f >> 1]);
+    });
+    Geo g;
+    g.x = gf[G_X]; g.y = gf[G_Y]; g.z = gf[G_Z]; g.f = gf[G_F]; g.ghm = gf[G_GHM];
+    g.pu0 = gf[G_PU]; g.pu1 = gf[G_PU + 1]; g.pu2 = gf[G_PU + 2];
+    g.pv0 = gf[G_PV]; g.pv1 = gf[G_PV + 1]; g.pv2 = gf[G_PV + 2];
+    g.pw0 = gf[G_PW]; g.pw1 = gf[G_PW + 1]; g.pw2 = gf[G_PW + 2];
+    g.gx = gf[G_GG]; g.gy = gf[G_GG + 1]; g.gz = gf[G_GG + 2];
+    double F[4];
+    rhs_point(S, u, v, w, h, g, m.gh0, F);
+    if (live)
+        stage_update_pair<STAGE, EXACT>(st, 4L * node + 2 * k, k ? F[2] : F[0], k ? F[3] : F[1]);
+}
+
 // Wave-uniform load through the constant address space: read-only, so it
 // stays a scalar load even after stores the compiler cannot rule out as
 // aliases.
@@ -968,16 +1108,6 @@ __device__ __forceinline__ TmemItem tmem_item(const DevTiles &tl, long it, int w
 // LDS image of one item (doubles): weights [4][2][32][4], geometry
 // [4][2][32], rows [kTmemRows][2][32][2].
 constexpr int kTmemItemDoubles = kTmemWtsDoubles + kTmemGeoDoubles + 128 * kTmemRows;
-
-// Global loads that fill an item's LDS image (wave slice wid's share),
-// held in registers until written.  gx is lane (half, k)'s 8-byte share of
-// its node's geometry row: field k for k < 17, stencil positions
-// 4 (k - 24) .. 4 (k - 24) + 3 for k >= 24 (zero for 17..23).
-struct TmemFill {
-    double2 fx[kTmemFill];
-    double2 xa, xb;
-    double gx;
-};
 
 // Geometry-row slot of the 32 16-bit stencil positions (8 doubles).
 constexpr int kTmemPosSlot = 24;
@@ -1046,51 +1176,41 @@ __device__ __forceinline__ TmemLoaderItem tmem_loader_item(const DevTiles &tl, l
     return L;
 }
 
-template <int NNBR>
-__device__ __forceinline__ TmemFill tmem_fill_load(const DevMesh &m, const DevTiles &tl,
-                                                   const double *__restrict__ src, long B,
-                                                   const TmemItem &I, const int *ids, int lane)
-{
-    const int fv = lane >> 4;
-    const int fp = lane & 15;
-    TmemFill L;
-    // node-major copies: a node's weights are one contiguous 1 KB wave load,
-    // its geometry and positions one 256-byte half-wave load
-    L.xa = *reinterpret_cast<const double2 *>(tl.wts_node + 128L * I.nA + 2 * lane);
-    L.xb = *reinterpret_cast<const double2 *>(tl.wts_node + 128L * I.nB + 2 * lane);
-    const int node = (lane >> 5) ? I.nB : I.nA;
-    L.gx = tl.geo_node[32L * node + (lane & 31)];
-    const double *__restrict__ s0 = src + I.col0 + 2 * fp + (long)fv * B;
-    // row ids at a fixed per-tile stride; the zero padding past the tile's
-    // count is a readable node id, loaded but not stored
-#pragma unroll
-    for (int q = 0; q < kTmemFill; ++q)
-        L.fx[q] = *reinterpret_cast<const double2 *>(s0 + 4L * ids[q] * B);
-    return L;
-}
-
-__device__ __forceinline__ void tmem_fill_store(double *img, const TmemItem &I,
-                                                const TmemFill &L, const TmemItem &next, int wid,
-                                                int lane)
+// Issues wave slice wid's share of an item's LDS image as direct-to-LDS
+// loads: its kTmemFill tile rows, the weights of its two nodes and their
+// geometry rows (lanes 0..31, 16 bytes each).  The header slots of the
+// geometry row are overwritten by tmem_dma_header once these loads are
+// complete.
+__device__ __forceinline__ void tmem_dma_issue(double *img, const DevTiles &tl,
+                                               const double *__restrict__ src, long B,
+                                               const TmemItem &I, const int *ids, int wid,
+                                               int lane)
 {
     double *wl = img;
     double *gl = img + kTmemWtsDoubles;
     double *rows = gl + kTmemGeoDoubles;
-    double2 *dst = reinterpret_cast<double2 *>(wl) + wid * 128 + lane;
-    dst[0] = L.xa;    // [wid][0][slot][..]
-    dst[64] = L.xb;   // [wid][1][slot][..]
-    // [wid][half][geometry 0..16 | next-item header 17, 18 (half 0) | positions 24..31]
-    const double hw = tmem_hdr_word(next, lane == kTmemHdrSlot + 1);
-    gl[wid * 64 + lane] = (lane == kTmemHdrSlot || lane == kTmemHdrSlot + 1) ? hw : L.gx;
-    // rows [row][var][32 members] as in global memory: lane (var fv, fp)
-    // writes members 2 fp, 2 fp + 1 of variable fv, 16 contiguous bytes
-    double2 *d0 = reinterpret_cast<double2 *>(rows) + lane;
+    dma16(tl.wts_node + 128L * I.nA + 2 * lane, wl + wid * 256);
+    dma16(tl.wts_node + 128L * I.nB + 2 * lane, wl + wid * 256 + 128);
+    if (lane < 32)
+        dma16(tl.geo_node + 32L * (lane < 16 ? I.nA : I.nB) + 2 * (lane & 15), gl + wid * 64);
+    // rows [row][var][32 members]: lane (var fv, fp) loads members 2 fp, 2 fp + 1
+    const double *__restrict__ s0 = src + I.col0 + 2 * (lane & 15) + (long)(lane >> 4) * B;
 #pragma unroll
     for (int q = 0; q < kTmemFill; ++q) {
         const int rr = wid * kTmemFill + q;
-        if (rr < I.ucount)   // a fresh double2 keeps fx in registers (one b128 write)
-            d0[rr * 64] = make_double2(L.fx[q].x, L.fx[q].y);
+        if (rr < I.ucount)   // wave-uniform
+            dma16(s0 + 4L * ids[q] * B, rows + rr * 128);
     }
+}
+
+// The next-item header (geometry slots kTmemHdrSlot, +1 of wave slice wid's
+// first row), written after the slice's direct-to-LDS loads completed.
+__device__ __forceinline__ void tmem_dma_header(double *img, const TmemItem &next, int wid,
+                                                int lane)
+{
+    double *gl = img + kTmemWtsDoubles;
+    if (lane == kTmemHdrSlot || lane == kTmemHdrSlot + 1)
+        gl[wid * 64 + lane] = tmem_hdr_word(next, lane == kTmemHdrSlot + 1);
 }
 
 // The epilogue's global operands H and D (the own state comes from the
@@ -1263,10 +1383,11 @@ __device__ __forceinline__ TmemRange tmem_range(const DevTiles &tl, int nblk)
 // The same items with specialised waves and two LDS images (160 KB, one
 // workgroup per CU): waves 4..7 fill one image while waves 0..3 compute
 // from the other, one barrier per item.  Loader wave 4 + w fills the share
-// of compute wave w and keeps two items in flight: it issues the loads of
-// item i + 2 before writing item i + 1 (loaded one barrier earlier) into
-// the free image.  A compute wave issues its own epilogue loads (own
-// state, H, D) before its stencil loop.
+// of compute wave w with direct-to-LDS loads (dma16: no register staging,
+// no LDS write instructions), one item ahead: right after the barrier that
+// frees an image it issues the next item's loads into it, waits for them
+// and writes the header naming the item after.  A compute wave issues its
+// own epilogue loads (H, D) one item ahead, before its stencil loop.
 template <int STAGE, bool EXACT, int NNBR>
 __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, DevStage st)
 {
@@ -1280,29 +1401,29 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
     const int active_members = c_i32(st.active);
     if (r.it0 >= r.it1)
         return;   // uniform over the workgroup
-    // loads are unconditional (a clamped, valid item past the end) so the
-    // wait before each use counts a fixed number of loads
+    // a clamped, valid item past the end keeps every scalar load in bounds
     auto clamp = [&](long i) { return i < r.it1 ? i : r.it0; };
     if (wv >= 4) {
-        const TmemLoaderItem L0 = tmem_loader_item(tl, r.it0, wid, active_members);
-        const TmemItem I0 = L0.I;
-        const TmemFill f0 = tmem_fill_load<NNBR>(m, tl, st.src, B, I0, L0.ids, lane);
-        const TmemLoaderItem L1 = tmem_loader_item(tl, clamp(r.it0 + r.W), wid, active_members);
-        TmemItem nI = L1.I;
-        TmemFill nf = tmem_fill_load<NNBR>(m, tl, st.src, B, nI, L1.ids, lane);
-        tmem_fill_store(lds, I0, f0, nI, wid, lane);
+        TmemLoaderItem L = tmem_loader_item(tl, r.it0, wid, active_members);
+        tmem_dma_issue(lds, tl, st.src, B, L.I, L.ids, wid, lane);
+        TmemLoaderItem Ln = tmem_loader_item(tl, clamp(r.it0 + r.W), wid, active_members);
+        __builtin_amdgcn_s_waitcnt(0);
+        tmem_dma_header(lds, Ln.I, wid, lane);
         __syncthreads();
         int b = 0;
         for (long it = r.it0; it < r.it1; it += r.W) {
+            // image b ^ 1 is free: fill it with item it + W
+            const bool more = it + r.W < r.it1;
+            if (more)
+                tmem_dma_issue(lds + (b ^ 1) * kTmemItemDoubles, tl, st.src, B, Ln.I, Ln.ids,
+                               wid, lane);
             const TmemLoaderItem L2 = tmem_loader_item(tl, clamp(it + 2 * r.W), wid,
                                                        active_members);
-            const TmemItem I2 = L2.I;
-            const TmemFill f2 = tmem_fill_load<NNBR>(m, tl, st.src, B, I2, L2.ids, lane);
-            if (it + r.W < r.it1)
-                tmem_fill_store(lds + (b ^ 1) * kTmemItemDoubles, nI, nf, I2, wid, lane);
+            __builtin_amdgcn_s_waitcnt(0);
+            if (more)
+                tmem_dma_header(lds + (b ^ 1) * kTmemItemDoubles, L2.I, wid, lane);
             __syncthreads();   // image b is free, image b ^ 1 is filled
-            nI = I2;
-            nf = f2;
+            Ln = L2;
             b ^= 1;
         }
     } else {

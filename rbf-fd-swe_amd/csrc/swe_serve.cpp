@@ -177,7 +177,7 @@ int new_batch(swe_srv *s, int mesh_id, int *batch_id)
         }
     swe_hip_options o{s->cfg.slots_per_batch, s->cfg.flags, s->cfg.device, s->cfg.member_block,
                       owner, 0, 0, m.tiled ? SWE_HIP_ORDER_HILBERT : SWE_HIP_ORDER_HOST,
-                      m.tiled ? 1 : 0, 0};
+                      m.tiled ? 1 : 0, 0, 0, 0};
     swe_hip_ctx *ctx = nullptr;
     int rc = swe_hip_create_ex(owner ? nullptr : &m.desc, &o, &ctx);
     if (rc == SWE_HIP_ENOMEM)

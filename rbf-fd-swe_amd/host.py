@@ -83,6 +83,8 @@ def lib() -> C.CDLL:
         L = C.CDLL(path)
         L.swe_problem_load.argtypes = [C.c_char_p, C.c_int, C.POINTER(Layout),
                                        C.POINTER(ProblemStruct)]
+        L.swe_problem_load_ex.argtypes = [C.c_char_p, C.c_int, C.POINTER(Layout), C.c_int,
+                                          C.POINTER(ProblemStruct)]
         L.swe_problem_free.argtypes = [C.POINTER(ProblemStruct)]
         L.swe_verify_output.argtypes = [C.POINTER(Patch), C.POINTER(Layout), _dp, _dp, C.c_double]
         L.swe_rcm_mapping.argtypes = [C.POINTER(Gsmd), C.POINTER(_ip)]
@@ -117,12 +119,13 @@ class Problem:
     C host prepared; the C allocation is released before returning)."""
 
     def __init__(self, path: str, use_rcm: bool = True, layout: str = "cfdl",
-                 simd: int = 4):
+                 simd: int = 4, fp32: bool = False):
         self._s = ProblemStruct()
         self.layout_name = layout
         self.layout = make_layout(layout, simd)
-        err = lib().swe_problem_load(path.encode(), int(bool(use_rcm)),
-                                     C.byref(self.layout), C.byref(self._s))
+        self.fp32 = bool(fp32)   # values rounded as the reference's FP32 build reads them
+        err = lib().swe_problem_load_ex(path.encode(), int(bool(use_rcm)),
+                                        C.byref(self.layout), int(self.fp32), C.byref(self._s))
         if err:
             raise RuntimeError(f"swe_problem_load({path}) failed with {err}")
         g, p = self._s.g, self._s.patch

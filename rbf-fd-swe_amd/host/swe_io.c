@@ -60,8 +60,19 @@ void swe_gsmd_free(swe_gsmd *g)
     memset(g, 0, sizeof(*g));
 }
 
+/* The reference's FP32 model type (RUN_32BIT, swe_config.h:75-80) rounds
+ * every value it reads to float ((fType)tmp, input.c:211-306); values are
+ * kept here as the doubles of those floats. */
+static void round_f32(double *a, size_t count)
+{
+    for (size_t k = 0; k < count; k++)
+        a[k] = (double)(float)a[k];
+}
+
 /* get_GSMD_bin, input.c:189-332. */
-int swe_get_gsmd_bin(const char *path, swe_gsmd *g)
+int swe_get_gsmd_bin(const char *path, swe_gsmd *g) { return swe_get_gsmd_bin_ex(path, 0, g); }
+
+int swe_get_gsmd_bin_ex(const char *path, int fp32, swe_gsmd *g)
 {
     memset(g, 0, sizeof(*g));
     FILE *fp = fopen(path, "rb");
@@ -101,13 +112,29 @@ int swe_get_gsmd_bin(const char *path, swe_gsmd *g)
         swe_gsmd_free(g);
         return err;
     }
+    if (fp32) {
+        for (int k = 0; k < 5; k++)
+            round_f32(*scal[k], N);
+        for (int k = 0; k < 4; k++)
+            round_f32(*vec[k], 3 * N);
+        for (int k = 0; k < 4; k++)
+            round_f32(*dm[k], NW);
+    }
     /* Unit-sphere weights -> Earth radius and hyperviscosity coefficient,
-     * done in double as the reference does (input.c:314-319). */
+     * done in double with the file's a and gamma as the reference does
+     * (input.c:314-319); FP32 rounds the result on the store. */
     for (size_t k = 0; k < NW; k++) {
         g->Dx[k] = g->Dx[k] / g->a;
         g->Dy[k] = g->Dy[k] / g->a;
         g->Dz[k] = g->Dz[k] / g->a;
         g->L[k] = g->L[k] * g->gamma;
+    }
+    if (fp32) {
+        for (int k = 0; k < 4; k++)
+            round_f32(*dm[k], NW);
+        double *cst[4] = {&g->a, &g->gh0, &g->dt, &g->gamma};   /* GSMD.a = (fType)a ... */
+        for (int k = 0; k < 4; k++)
+            round_f32(cst[k], 1);
     }
     /* 1-based -> 0-based (input.c:326-328); reject ids outside the mesh. */
     for (size_t k = 0; k < NW; k++) {
@@ -150,3 +177,20 @@ static int read_state(const char *path, int which, double **out)
 
 int swe_get_ics_bin(const char *path, double **H_init) { return read_state(path, 0, H_init); }
 int swe_get_fcs_bin(const char *path, double **H_100) { return read_state(path, 1, H_100); }
+
+/* The same with the FP32 model type's rounding (input.c:369-371, :406-408). */
+int swe_get_state_bin_ex(const char *path, int which, int fp32, double **H)
+{
+    int err = read_state(path, which, H);
+    if (!err && fp32) {
+        int N, W;
+        FILE *fp = fopen(path, "rb");
+        if (!fp)
+            return SWE_HOST_EOPEN;
+        err = read_header(fp, &N, &W);
+        fclose(fp);
+        if (!err)
+            round_f32(*H, 4 * (size_t)N);
+    }
+    return err;
+}

@@ -244,15 +244,21 @@ void swe_problem_free(swe_problem *p)
 int swe_problem_load(const char *path, int use_rcm, const swe_layout *l,
                      swe_problem *p)
 {
+    return swe_problem_load_ex(path, use_rcm, l, 0, p);
+}
+
+int swe_problem_load_ex(const char *path, int use_rcm, const swe_layout *l, int fp32,
+                        swe_problem *p)
+{
     memset(p, 0, sizeof(*p));
     if (!l || l->simd <= 0)
         return SWE_HOST_EARG;
     p->layout = *l;
-    int err = swe_get_gsmd_bin(path, &p->g);
+    int err = swe_get_gsmd_bin_ex(path, fp32, &p->g);
     if (!err)
-        err = swe_get_ics_bin(path, &p->H_init);
+        err = swe_get_state_bin_ex(path, 0, fp32, &p->H_init);
     if (!err)
-        err = swe_get_fcs_bin(path, &p->H_100);
+        err = swe_get_state_bin_ex(path, 1, fp32, &p->H_100);
     if (!err && use_rcm) {
         err = swe_rcm_mapping(&p->g, &p->mapping);
         if (!err)

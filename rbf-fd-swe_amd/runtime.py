@@ -15,7 +15,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 SWE_HIP_EXACT = 1
-KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4}
+KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4, "tile": 5}
 ORDERS = {"host": 0, "morton": 1, "hilbert": 2}
 
 # Every function include/swe_hip.h declares.
@@ -45,7 +45,7 @@ class SweHipOptions(C.Structure):
     _fields_ = [("members", C.c_int), ("flags", C.c_uint), ("device", C.c_int),
                 ("member_block", C.c_int), ("share", C.c_void_p), ("kernel", C.c_int),
                 ("prefetch", C.c_int), ("node_order", C.c_int), ("tiled", C.c_int),
-                ("slot_block", C.c_int)]
+                ("slot_block", C.c_int), ("tile_nodes", C.c_int), ("tile_rows", C.c_int)]
 
 
 class SweHipTimers(C.Structure):
@@ -147,14 +147,15 @@ class HipContext:
     def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0,
                  member_block: int = 0, share: "HipContext | None" = None,
                  kernel: str = "auto", prefetch: int = 0, order: str = "host",
-                 tiled: int = 0, slot_block: int = 0):
+                 tiled: int = 0, slot_block: int = 0, tile_nodes: int = 0,
+                 tile_rows: int = 0):
         self.prob = prob
         self.members = int(members)
         self.exact = bool(exact)
         opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
                              int(member_block), share.handle if share is not None else None,
                              KERNELS[kernel], int(prefetch), ORDERS[order], int(tiled),
-                             int(slot_block))
+                             int(slot_block), int(tile_nodes), int(tile_rows))
         h = C.c_void_p()
         if share is not None:
             rc = lib().swe_hip_create_ex(None, C.byref(opts), C.byref(h))

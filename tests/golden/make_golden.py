@@ -14,6 +14,10 @@ Outputs (all data, no reference source):
                     order: F0 = eval_RHS_swe(H_init), H1/H10 = states after
                     1 and 10 steps, rcm = rcm_mapping(), plus the
                     reference's 2562 input SHA-256 and its H_100 check count
+  icos642_ref32.npz the same vectors (float32) from the reference built with
+                    its header as shipped (RUN_32BIT, fType = float), plus
+                    H100, on the committed icos642_case.bin
+                    (python tests/golden/make_golden.py fp32 writes only this)
 """
 import hashlib
 import os
@@ -60,5 +64,20 @@ def main():
     print("wrote", tmp, "max|u| after 100 steps", np.abs(H100[:, :3]).max())
 
 
+def main_fp32():
+    path = os.path.join(HERE, "icos642_case.bin")
+    ref = pyoracle.RefBuild(path, use_rcm=True, layout="cfdl", fp32=True)
+    out = {"F0": natural(ref, ref.eval_rhs(ref.H_init))}
+    for k in (1, 10, 100):
+        out[f"H{k}"] = natural(ref, ref.run(ref.H_init, k))
+    assert all(v.dtype == np.float32 for v in out.values())
+    np.savez_compressed(os.path.join(HERE, "icos642_ref32.npz"),
+                        rcm=ref.mapping.astype(np.int32), **out)
+    print("wrote icos642_ref32.npz, max|u| after 100 steps", np.abs(out["H100"][:, :3]).max())
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "fp32":
+        main_fp32()
+    else:
+        main()

@@ -75,9 +75,10 @@ def test_bad_options_rejected_before_device(pkg):
     L = pkg.runtime.lib()
     base = dict(members=1, flags=0, device=0, member_block=0, share=None, kernel=0,
                 prefetch=0, node_order=0, tiled=0, slot_block=0)
-    for field, bad in [("kernel", 5), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
+    for field, bad in [("kernel", 6), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
                        ("slot_block", 8), ("members", 0), ("member_block", 3),
-                       ("tiled", 2), ("tiled", -1)]:
+                       ("tiled", 2), ("tiled", -1), ("tile_nodes", 100), ("tile_nodes", -2),
+                       ("tile_rows", 8), ("tile_rows", 71)]:
         o = pkg.runtime.SweHipOptions(**dict(base, **{field: bad}))
         if field == "member_block":
             o.members = 8

@@ -56,3 +56,20 @@ def test_other_layouts_same_numbers(pkg, pyoracle, golden):
         prob = pkg.host.Problem(CASE, use_rcm=True, layout=lay, simd=8)
         H = pyoracle.Oracle(prob).run(prob.H_init, 10)
         assert np.array_equal(natural(prob, H), g["H10"]), lay
+
+
+REFV32 = os.path.join(GOLDEN, "icos642_ref32.npz")
+
+
+@pytest.mark.parametrize("layout", ["cfdl", "sfdl"])
+def test_fp32_oracle_matches_reference_fixtures(pkg, pyoracle, layout):
+    """The FP32 oracle against the vectors the reference's shipped FP32 build
+    wrote (RHS, 1, 10 and 100 steps, float32, natural order), and the FP32
+    known-answer check against the file's H_100 at 4e-2."""
+    g = np.load(REFV32, allow_pickle=False)
+    prob = pkg.host.Problem(CASE, use_rcm=True, layout=layout, simd=4, fp32=True)
+    orc = pyoracle.Oracle(prob, "strict32")
+    assert np.array_equal(natural(prob, orc.eval_rhs(prob.H_init)), g["F0"])
+    for k in (1, 10, 100):
+        assert np.array_equal(natural(prob, orc.run(prob.H_init, k)), g[f"H{k}"]), k
+    assert orc.verify(orc.run(prob.H_init, 100), prob.H_100, 4e-2) == 0

@@ -193,7 +193,8 @@ def test_tiled_fast_composition_and_active(pkg, mb, order):
                                                  ("quadw", 0, 0, "morton"),
                                                  ("quadw", 4, 16, "morton"),
                                                  ("pair", 0, 0, "morton"), ("pair", 0, 16, "morton"),
-                                                 ("pair", 0, 0, "hilbert")])
+                                                 ("pair", 0, 0, "hilbert"), ("tile", 0, 0, "hilbert"),
+                                                 ("tile", 0, 16, "morton"), ("tile", 0, 0, "host")])
 def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk, order):
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
     print(f"case: single-state {kernel} prefetch={pf} slot_block={sblk} {order} exact", flush=True)
@@ -207,7 +208,8 @@ def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk, order):
 
 @pytest.mark.parametrize("kernel,order,sblk", [("pair", "host", 0), ("pair", "morton", 16),
                                                ("pair", "hilbert", 0), ("quadw", "morton", 16),
-                                               ("lane", "host", 0)])
+                                               ("lane", "host", 0), ("tile", "hilbert", 0),
+                                               ("tile", "host", 16)])
 def test_single_state_variants_fast(pkg, pyoracle, kernel, order, sblk):
     """Fast single-state kernels within 1e-12 of the oracle after 50 steps."""
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
@@ -235,3 +237,41 @@ def test_blocked_stencil_layout_members_exact(pkg, pyoracle, tiled, mb):
         ctx.rk_steps(3)
         for m in (0, 33, 63):
             assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 3)), m
+
+
+@pytest.mark.parametrize("n", [642, 10242])
+def test_tile_kernel_fast_equals_pair_and_drop_in(pkg, pyoracle, n):
+    """The LDS-tile single-state kernel runs the pair kernel's arithmetic:
+    fused fast steps bitwise equal to the pair kernel's, and the drop-in
+    substeps and a plain RHS bitwise equal to the oracle in exact mode."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(n, seed=1), use_rcm=True, layout="sfdl")
+    print(f"case: tile kernel N={n} vs pair (fast) and oracle (exact substeps)", flush=True)
+    with pkg.runtime.HipContext(prob, kernel="tile", order="hilbert") as t, \
+            pkg.runtime.HipContext(prob, kernel="pair", order="hilbert") as p:
+        for c in (t, p):
+            c.set_state(prob.H_init)
+            c.rk_steps(9)
+        assert np.array_equal(t.get_state(), p.get_state())
+    orc = pyoracle.Oracle(prob)
+    with pkg.runtime.HipContext(prob, kernel="tile", order="hilbert", exact=True) as t:
+        assert np.array_equal(t.eval_rhs(prob.H_init), orc.eval_rhs(prob.H_init))
+        t.set_state(prob.H_init)
+        for s in range(4):
+            t.rk_substep(s)
+        t.rk_steps(1)
+        assert np.array_equal(t.get_state(), orc.run(prob.H_init, 2))
+
+
+@pytest.mark.parametrize("tn,pf", [(64, 0), (64, 4), (128, 8), (256, 4), (256, 8)])
+def test_tile_kernel_shapes_and_staging_exact(pkg, pyoracle, tn, pf):
+    """Every tile size and row-staging schedule of the LDS-tile single-state
+    kernel gives the oracle's bits (fused steps and the drop-in RHS)."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="cfdl")
+    print(f"case: tile kernel tile_nodes={tn} prefetch={pf} exact", flush=True)
+    orc = pyoracle.Oracle(prob)
+    with pkg.runtime.HipContext(prob, exact=True, kernel="tile", order="hilbert",
+                                tile_nodes=tn, prefetch=pf) as ctx:
+        assert np.array_equal(ctx.eval_rhs(prob.H_init), orc.eval_rhs(prob.H_init))
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(3)
+        assert np.array_equal(ctx.get_state(), orc.run(prob.H_init, 3))

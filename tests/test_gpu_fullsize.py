@@ -8,12 +8,13 @@ from conftest import rel_by_var
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n,steps", [(40962, 1000), (655362, 200)])
-def test_single_member_full_size(pkg, pyoracle, n, steps):
-    print(f"case: full size N={n} steps={steps}", flush=True)
+@pytest.mark.parametrize("n,steps,kernel", [(40962, 1000, "auto"), (655362, 200, "auto"),
+                                            (655362, 200, "tile")])
+def test_single_member_full_size(pkg, pyoracle, n, steps, kernel):
+    print(f"case: full size N={n} steps={steps} kernel={kernel}", flush=True)
     prob = pkg.host.Problem(pkg.mesh.case_path(n, seed=1), use_rcm=True, layout="cfdl")
     want = pyoracle.Oracle(prob, "strict").run(prob.H_init, steps)
-    with pkg.runtime.HipContext(prob) as ctx:
+    with pkg.runtime.HipContext(prob, kernel=kernel, order="hilbert") as ctx:
         ctx.set_state(prob.H_init)
         ctx.rk_steps(steps)
         got = ctx.get_state()

@@ -109,3 +109,65 @@ def test_profiling_report_matches_reference(pkg, pyoracle, nattempts):
         assert [f"{v:e}" for v in ours[k]] == [f"{v:e}" for v in ref[k]], (k, ours[k], ref[k])
     if nattempts <= 2:
         assert (ours[1, :3] == 0.0).all() and ours[1, 3] > 0.0
+
+
+@pytest.fixture(scope="module", params=[(lay, rcm) for lay in LAYOUTS for rcm in (True, False)],
+                ids=lambda p: f"{p[0]}-rcm{int(p[1])}")
+def pair32(request, pkg, pyoracle):
+    """The reference's shipped FP32 build (RUN_32BIT, swe_config.h:75) and
+    the C host's FP32 reading of the same input."""
+    lay, rcm = request.param
+    if not pyoracle.ref_available(lay, fp32=True):
+        pytest.skip("FP32 reference build missing")
+    ref = pyoracle.RefBuild(REFERENCE_BIN, use_rcm=rcm, layout=lay, fp32=True)
+    prob = pkg.host.Problem(REFERENCE_BIN, use_rcm=rcm, layout=lay, simd=4, fp32=True)
+    return ref, prob
+
+
+def test_fp32_host_prep_bitwise(pair32):
+    """Every prepared array of the FP32 reading (rounding of input.c:211-319:
+    values to float, weights scaled in double and rounded again) equals the
+    FP32 reference build's, as floats."""
+    ref, prob = pair32
+    for nm in ARRAYS:
+        got = getattr(prob, nm)
+        want = getattr(ref, nm)
+        if nm != "idx":
+            assert np.array_equal(got.astype(np.float32).astype(np.float64), got), nm
+            got = got.astype(np.float32)
+        assert np.array_equal(got, want), nm
+    assert np.float32(prob.gh0) == np.float32(ref.gh0) and np.float32(prob.dt) == np.float32(ref.dt)
+
+
+def test_fp32_oracle_bitwise_and_known_answer(pair32, pyoracle):
+    """The FP32 oracle equals the FP32 reference build bit for bit (RHS and 100
+    steps) and passes the reference's FP32 known-answer check (4e-2,
+    swe_config.h:80)."""
+    ref, prob = pair32
+    orc = pyoracle.Oracle(prob, "strict32")
+    assert np.array_equal(orc.eval_rhs(prob.H_init), ref.eval_rhs(ref.H_init))
+    H = orc.run(prob.H_init, 100)
+    assert H.dtype == np.float32
+    assert np.array_equal(H, ref.run(ref.H_init, 100))
+    assert orc.verify(H, prob.H_100, 4e-2) == 0
+
+
+def test_fp32_update_h_is_double_rounded_once(pkg, pyoracle):
+    """Planted-mistake check of the FP32 stage-3 update: computing
+    H += (dt/6)*D in float instead of the reference's double (the C promotion
+    of rk4_rbffd_swe.c:115) changes the trajectory, so the bitwise test above
+    would catch it."""
+    prob = pkg.host.Problem(REFERENCE_BIN, use_rcm=True, layout="cfdl", fp32=True)
+    orc = pyoracle.Oracle(prob, "strict32")
+    H = prob.H_init.astype(np.float32)
+    for s in range(3):
+        orc.rk_substep(H, s)
+    D = orc.D.copy()
+    orc.rk_substep(H, 3)
+    dt = np.float32(prob.dt)
+    H0 = prob.H_init.astype(np.float32)
+    Df = D + orc.F   # update_D with coeff 1 (float)
+    H_double = (H0.astype(np.float64) + (np.float64(dt) / 6.0) * Df.astype(np.float64)).astype(np.float32)
+    H_float = H0 + np.float32(np.float64(dt) / 6.0) * Df
+    assert np.array_equal(H, H_double)
+    assert not np.array_equal(H_double, H_float)
