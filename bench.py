@@ -41,9 +41,15 @@ NNBR = 31
 # (64 + 128 + 128 + 96) / 4 = 104 B).  Per member beyond the first: 32 + 104.
 STATIC_B = NNBR * (4 + 4 * 8) + 17 * 8
 STATE_B = 32 + 104
+# The same for the FP32 model type: 31*(4 + 4*4) + 17*4 = 688 B static,
+# 16 B own state, (32 + 64 + 64 + 48) / 4 = 52 B stage epilogue.
+STATIC_B32 = NNBR * (4 + 4 * 4) + 17 * 4
+STATE_B32 = 16 + 52
 
 
-def stage_bytes(n_nodes: int, members: int) -> float:
+def stage_bytes(n_nodes: int, members: int, fp32: bool = False) -> float:
+    if fp32:
+        return float(n_nodes) * (STATIC_B32 + STATE_B32 * members)
     return float(n_nodes) * (STATIC_B + STATE_B * members)
 
 
@@ -66,6 +72,8 @@ def parse_args():
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--layout", default="cfdl")
     ap.add_argument("--exact", action="store_true")
+    ap.add_argument("--fp32", action="store_true",
+                    help="the reference's FP32 model type (SWE_HIP_FP32 context) for the main leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 4 device copies of 1 GiB (HBM counter calibration) and exit")
@@ -267,14 +275,14 @@ def main():
 
     t0 = time.time()
     path = pkg.mesh.case_path(args.nodes, seed=1)
-    prob = pkg.host.Problem(path, use_rcm=True, layout=args.layout, simd=4)
+    prob = pkg.host.Problem(path, use_rcm=True, layout=args.layout, simd=4, fp32=args.fp32)
     print(f"bench: input ready N={prob.n_nodes} ({time.time() - t0:.1f}s)", file=sys.stderr,
           flush=True)
     ctx = pkg.runtime.HipContext(prob, members=args.members, exact=args.exact,
                                  member_block=args.member_block, kernel=args.kernel,
                                  prefetch=args.prefetch, order=args.order, tiled=args.tiled,
                                  slot_block=args.slot_block, tile_nodes=args.tile_nodes,
-                                 tile_rows=args.tile_rows)
+                                 tile_rows=args.tile_rows, fp32=args.fp32)
     for m in range(args.members):
         ctx.set_state(prob.H_init, m)
     stream = torch.cuda.ExternalStream(ctx.stream())
@@ -298,7 +306,7 @@ def main():
     N = prob.padded
     launches = 4 * args.steps
     launch_s = gpu_ms * 1e-3 / launches
-    achieved = stage_bytes(N, args.members) / launch_s / 1e9
+    achieved = stage_bytes(N, args.members, args.fp32) / launch_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_n{args.nodes}_b{args.members}.json")
     if os.path.exists(tpath):
@@ -339,7 +347,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": dominant_kernel(args),
-                     "bytes_per_launch": stage_bytes(N, args.members),
+                     "bytes_per_launch": stage_bytes(N, args.members, args.fp32),
                      "avg_launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,
         "ensemble": ensemble,

@@ -23,7 +23,8 @@
  * swe_hip_last_error() and exits.  Host buffers are caller-owned; device
  * buffers are owned by the context.  One host thread per context; work is
  * ordered on the context's stream and the functions that return data to the
- * host synchronise it.  FP64 only.
+ * host synchronise it.  FP64, or the reference's FP32 model type through
+ * SWE_HIP_FP32 contexts.
  */
 #ifndef SWE_HIP_H
 #define SWE_HIP_H
@@ -42,6 +43,13 @@ extern "C" {
 #define SWE_HIP_EXACT 1u      /* sequential stencil sums, no FMA contraction:
                                  bitwise equal to the reference CPU build
                                  compiled without contraction              */
+#define SWE_HIP_FP32 2u       /* the reference's FP32 model type (RUN_32BIT,
+                                 fType = float, swe_config.h:75-80): static
+                                 data and state held and computed in float,
+                                 update_H's dt/6.0 product in double as the
+                                 reference's C promotion does; one state
+                                 (members == 1); state I/O through the *32
+                                 entry points                              */
 
 /* The static model data of the local patch (PSMD_struct,
  * swe_code/main/include/swe_config.h:187-233) as plain pointers, in the
@@ -62,6 +70,18 @@ typedef struct swe_hip_static {
     const double *Dx, *Dy, *Dz, *L;            /* DM_LIN_ID layout, scaled */
 } swe_hip_static;
 
+/* The same static data in the reference's FP32 model type (its PSMD_struct
+ * with fType = float), for swe_hip_create32. */
+typedef struct swe_hip_static32 {
+    int compute_size, patch_size, compute_pid_s, Nnbr, padded_Nnbr;
+    int cfdl, sfdl, simd;
+    float gh0, dt;
+    const float *x, *y, *z, *f, *ghm;
+    const float *p_u, *p_v, *p_w, *gradghm;
+    const int *idx;
+    const float *Dx, *Dy, *Dz, *L;
+} swe_hip_static32;
+
 typedef struct swe_hip_ctx swe_hip_ctx;
 
 /* Per-phase device time accumulated by the drop-in path while timing is on,
@@ -79,6 +99,19 @@ typedef struct swe_hip_timers {
 int swe_hip_create(const swe_hip_static *s, int members, unsigned flags,
                    int device, swe_hip_ctx **out);
 void swe_hip_destroy(swe_hip_ctx *ctx);
+
+/* FP32 drop-in for the reference built as shipped (RUN_32BIT: the LPSMD
+ * arrays and the state are float): an SWE_HIP_FP32 context (members must be
+ * 1; flags may add SWE_HIP_EXACT).  Its state goes in and out through
+ * swe_hip_set_state32 / swe_hip_get_state32 (host layout SVM_LIN_ID, float)
+ * and its RHS hook is swe_hip_eval_rhs32; swe_hip_rk_substep and
+ * swe_hip_rk_steps advance it like any context.  The double entry points
+ * reject FP32 contexts and the *32 ones FP64 contexts (SWE_HIP_EINVAL). */
+int swe_hip_create32(const swe_hip_static32 *s, int members, unsigned flags, int device,
+                     swe_hip_ctx **out);
+int swe_hip_set_state32(swe_hip_ctx *ctx, int member, const float *H);
+int swe_hip_get_state32(swe_hip_ctx *ctx, int member, float *H);
+int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
 
 /* Extended creation.  members/flags/device as swe_hip_create; member_block
  * is the number of members one lane group of the ensemble kernel serves
@@ -179,7 +212,7 @@ int swe_hip_rk_steps(swe_hip_ctx *ctx, int nsteps);
  * swe_hip_rk_steps, launched without a graph; the stage states and the RK
  * sum live in a library-owned 64 MiB device array (one per GPU, held until
  * the process ends) instead of context buffers, so the call makes no device
- * allocation.  Limits: single-state contexts (members == 1) of
+ * allocation.  Limits: single-state FP64 contexts (members == 1) of
  * at most 699050 nodes (larger meshes fail with SWE_HIP_EINVAL).  Every
  * call on one GPU shares that array: calls that overlap on one GPU (other
  * threads, other streams) race. */

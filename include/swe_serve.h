@@ -28,6 +28,8 @@ extern "C" {
 
 typedef struct swe_srv swe_srv;
 
+#define SWE_SRV_EIO (-5)       /* a request's result file could not be written */
+
 typedef struct swe_srv_config {
     long long hbm_cap_bytes;   /* admission budget: static data + state buffers */
     int slots_per_batch;       /* member slots of one batched context (even)    */
@@ -58,11 +60,24 @@ void swe_srv_destroy(swe_srv *srv);
 /* Registers a mesh (static data in the host layout of swe_hip_static);
  * its device copy is made at the first admission and then kept. */
 int swe_srv_add_mesh(swe_srv *srv, const swe_hip_static *s, int *mesh_id);
+/* The same with the host -> file node mapping (the RCM mapping of the host's
+ * init pipeline, n_nodes real nodes; NULL = identity) that result files use. */
+int swe_srv_add_mesh_ex(swe_srv *srv, const swe_hip_static *s, const int *mapping, int n_nodes,
+                        int *mesh_id);
 
 /* Queues a request: init[m] / result[m] are host-layout states of member m
  * (patch_size*4 doubles each).  Returns its id in *req_id. */
 int swe_srv_submit(swe_srv *srv, int mesh_id, int members, const double *const *init,
                    double *const *result, int nsteps, long long tag, int *req_id);
+/* The same, and when result_path is not NULL the server also writes the
+ * request's result file when it completes (in swe_srv_pump): the history
+ * format of swe_run (the reference's USE_HIST output, runtime_params.c:24-27)
+ * -- int32 N, then per member an int32 step (= nsteps), f64 time and
+ * f64 H[N][4] (u, v, w, h) in the file node order of the mesh's mapping.
+ * A write failure makes swe_srv_pump return SWE_SRV_EIO. */
+int swe_srv_submit_ex(swe_srv *srv, int mesh_id, int members, const double *const *init,
+                      double *const *result, int nsteps, long long tag, const char *result_path,
+                      int *req_id);
 
 /* One non-blocking scheduling pass: retire members of batches whose chunk
  * finished, admit queued members, launch the next chunks. *progress is set

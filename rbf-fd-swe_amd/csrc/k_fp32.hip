@@ -1,0 +1,43 @@
+// Launchers of the FP32 single-state stage kernel.
+#include "swe_ctx.h"
+
+namespace swe {
+namespace {
+
+template <int STAGE, bool EX>
+void launch32(const swe_hip_ctx *c, DevStage32 st)
+{
+    const DevMesh32 m = mesh32_of(c);
+    // one-wave workgroups: at N = 40962 one lane per node is 641 waves, which
+    // 256-thread workgroups would put on 161 of the 256 CUs
+    st.nblk = blocks_x(c, (long)c->n, kSingleBlock);
+    const dim3 g(st.nblk), b(kSingleBlock);
+    if (c->nnbr == 31)
+        k_stage_lane32<STAGE, EX, 31><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_lane32<STAGE, EX, 0><<<g, b, 0, c->stream>>>(m, st);
+}
+
+template <int STAGE>
+void by_flags(const swe_hip_ctx *c, const DevStage32 &st)
+{
+    if (exact(c))
+        launch32<STAGE, true>(c, st);
+    else
+        launch32<STAGE, false>(c, st);
+}
+
+}  // namespace
+
+void launch_fp32_stage(const swe_hip_ctx *c, int stage, const DevStage32 &st)
+{
+    switch (stage) {
+    case ST_0: by_flags<ST_0>(c, st); break;
+    case ST_1: by_flags<ST_1>(c, st); break;
+    case ST_2: by_flags<ST_2>(c, st); break;
+    case ST_3: by_flags<ST_3>(c, st); break;
+    default: by_flags<ST_RHS>(c, st); break;
+    }
+}
+
+}  // namespace swe

@@ -4,6 +4,7 @@
 
 #include "swe_hip.h"
 #include "swe_kernels.h"
+#include "swe_kernels32.h"
 
 #include <hip/hip_runtime.h>
 
@@ -51,6 +52,8 @@ struct SweMesh {
     swe::HostLayout lay{1, 0, 1};
     int *d_nbr = nullptr;
     double *d_wts = nullptr, *d_geo = nullptr;
+    float *d_wts32 = nullptr, *d_geo32 = nullptr;   // FP32 meshes (d_wts, d_geo unused)
+    bool fp32 = false;
     long long bytes = 0;
     int order = SWE_HIP_ORDER_HOST;
     int sblk = 0;                 // stencil layout: 0 slot-major, 16 node blocks
@@ -88,6 +91,9 @@ struct swe_hip_ctx {
     int ncu = 0;                        // compute units of the device
     unsigned flags = 0;
     double *d_H = nullptr, *d_Ka = nullptr, *d_Kb = nullptr, *d_D = nullptr, *d_F = nullptr;
+    // FP32 contexts (SWE_HIP_FP32): the same buffers in float, d_* unused
+    bool fp32 = false;
+    float *f_H = nullptr, *f_Ka = nullptr, *f_Kb = nullptr, *f_D = nullptr, *f_F = nullptr;
     double *d_stage = nullptr;   // [n*4] staging for member upload/read-back
     int *d_active = nullptr;     // member slots advanced by the ensemble kernel
     SweTiles tiles;              // LDS-staged ensemble kernel (ntiles == 0: unused)
@@ -132,6 +138,23 @@ int set_error(int code, const std::string &msg);
 // Looks up this device's no-allocation scratch address (swe_noalloc.hip);
 // the device must be current.  On failure *err says why.
 int noalloc_setup(int dev, std::string *err);
+
+inline DevMesh32 mesh32_of(const swe_hip_ctx *c)
+{
+    DevMesh32 m;
+    m.nbr = c->d_nbr;
+    m.wts = c->mesh->d_wts32;
+    m.geo = c->mesh->d_geo32;
+    m.n = c->n;
+    m.ld = c->ld;
+    m.nnbr = c->nnbr;
+    m.sblk = c->sblk;
+    m.gh0 = (float)c->gh0;
+    return m;
+}
+
+// One FP32 stage (ST_0..ST_3) or RHS (ST_RHS) on the context's stream.
+void launch_fp32_stage(const swe_hip_ctx *c, int stage, const DevStage32 &st);
 
 // One fused stage (ST_0..ST_3) or a plain RHS (ST_RHS) on the context's
 // stream, by kernel family (each in its own translation unit).

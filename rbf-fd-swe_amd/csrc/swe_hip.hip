@@ -83,6 +83,43 @@ __global__ __launch_bounds__(256) void k_get_member(double *__restrict__ flat,
         flat[i] = state[i * B + member];
 }
 
+// Element-wise kernels of the FP32 drop-in stage driver over `count` values.
+__global__ __launch_bounds__(256) void k_copy32(float *__restrict__ dst,
+                                                const float *__restrict__ src, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        dst[i] = src[i];
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_axpy_inplace32(float *__restrict__ y, float a,
+                                                        const float *__restrict__ x, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        y[i] = axpy32<EXACT>(y[i], a, x[i]);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_axpy_to32(float *__restrict__ out,
+                                                   const float *__restrict__ y, float a,
+                                                   const float *__restrict__ x, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        out[i] = axpy32<EXACT>(y[i], a, x[i]);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_update_h32(float *__restrict__ H, double c6,
+                                                    const float *__restrict__ D, long count)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        H[i] = update_h32<EXACT>(H[i], c6, D[i]);
+}
+
 }  // namespace swe
 
 namespace {
@@ -119,8 +156,37 @@ namespace {
 
 // Fused stage s of one RK step: gathers from H (s=0), Ka (s=1), Kb (s=2),
 // Ka (s=3); writes Ka, Kb, Ka, H.
+// FP32 stage s of one RK step: the same buffers and constants in float;
+// stage 3 keeps dt/6.0 in double (update_H, rk4_rbffd_swe.c:86-121).
+void launch_fused32(const swe_hip_ctx *c, int s)
+{
+    DevStage32 st{};
+    st.h = c->f_H;
+    st.d = c->f_D;
+    const float dt = (float)c->dt;
+    switch (s) {
+    case 0:
+        st.src = c->f_H; st.out = c->f_Ka; st.c = (float)(dt / 2.0); st.dcoef = 0.0f;
+        break;
+    case 1:
+        st.src = c->f_Ka; st.out = c->f_Kb; st.c = (float)(dt / 2.0); st.dcoef = 2.0f;
+        break;
+    case 2:
+        st.src = c->f_Kb; st.out = c->f_Ka; st.c = dt; st.dcoef = 2.0f;
+        break;
+    default:
+        st.src = c->f_Ka; st.out = c->f_H; st.c6 = dt / 6.0; st.dcoef = 1.0f;
+        break;
+    }
+    launch_fp32_stage(c, s, st);
+}
+
 void launch_fused(const swe_hip_ctx *c, int s)
 {
+    if (c->fp32) {
+        launch_fused32(c, s);
+        return;
+    }
     DevStage st{};
     st.members = c->B;
     st.active = c->d_active;
@@ -215,6 +281,8 @@ void release_mesh(SweMesh *m)
     if (m->d_nbr) (void)hipFree(m->d_nbr);
     if (m->d_wts) (void)hipFree(m->d_wts);
     if (m->d_geo) (void)hipFree(m->d_geo);
+    if (m->d_wts32) (void)hipFree(m->d_wts32);
+    if (m->d_geo32) (void)hipFree(m->d_geo32);
     delete m;
 }
 
@@ -227,6 +295,7 @@ void free_ctx(swe_hip_ctx *c)
     for (hipGraphExec_t g : c->graphs)
         if (g) (void)hipGraphExecDestroy(g);
     void *ptrs[] = {c->d_H, c->d_Ka, c->d_Kb, c->d_D, c->d_F, c->d_stage, c->d_active,
+                    c->f_H, c->f_Ka, c->f_Kb, c->f_D, c->f_F,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
                     c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile,
                     c->tiles.d_wts_node, c->tiles.d_geo_node};
@@ -340,7 +409,8 @@ uint64_t hilbert3(double x, double y, double z)
     return key;
 }
 
-int build_mesh(const swe_hip_static *s, int device, int order, int sblk, SweMesh **out)
+int build_mesh(const swe_hip_static *s, int device, int order, int sblk, bool fp32,
+               SweMesh **out)
 {
     *out = nullptr;
     const int n = s->compute_size;
@@ -427,18 +497,33 @@ int build_mesh(const swe_hip_static *s, int device, int order, int sblk, SweMesh
     m->gh0 = s->gh0;
     m->dt = s->dt;
     m->lay = lay;
+    m->fp32 = fp32;
     int rc = 0;
-    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_nbr, sizeof(int) * upload_nbr.size());
-    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_wts, sizeof(double) * wts.size());
-    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_geo, sizeof(double) * geo.size());
     hipError_t e = hipSuccess;
+    if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_nbr, sizeof(int) * upload_nbr.size());
     if (!rc)
         e = hipMemcpy(m->d_nbr, upload_nbr.data(), sizeof(int) * upload_nbr.size(),
                       hipMemcpyHostToDevice);
-    if (!rc && e == hipSuccess)
-        e = hipMemcpy(m->d_wts, wts.data(), sizeof(double) * wts.size(), hipMemcpyHostToDevice);
-    if (!rc && e == hipSuccess)
-        e = hipMemcpy(m->d_geo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice);
+    if (fp32) {   // values rounded to float (exact for data read as FP32)
+        const std::vector<float> w32(wts.begin(), wts.end()), g32(geo.begin(), geo.end());
+        if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_wts32, sizeof(float) * w32.size());
+        if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_geo32, sizeof(float) * g32.size());
+        if (!rc && e == hipSuccess)
+            e = hipMemcpy(m->d_wts32, w32.data(), sizeof(float) * w32.size(),
+                          hipMemcpyHostToDevice);
+        if (!rc && e == hipSuccess)
+            e = hipMemcpy(m->d_geo32, g32.data(), sizeof(float) * g32.size(),
+                          hipMemcpyHostToDevice);
+    } else {
+        if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_wts, sizeof(double) * wts.size());
+        if (!rc) rc = dev_alloc(&m->bytes, (void **)&m->d_geo, sizeof(double) * geo.size());
+        if (!rc && e == hipSuccess)
+            e = hipMemcpy(m->d_wts, wts.data(), sizeof(double) * wts.size(),
+                          hipMemcpyHostToDevice);
+        if (!rc && e == hipSuccess)
+            e = hipMemcpy(m->d_geo, geo.data(), sizeof(double) * geo.size(),
+                          hipMemcpyHostToDevice);
+    }
     if (!rc && e != hipSuccess)
         rc = fail(SWE_HIP_EDEVICE, std::string("static data upload: ") + hipGetErrorString(e));
     if (rc) {
@@ -685,6 +770,14 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "node_order must be SWE_HIP_ORDER_HOST, _MORTON or _HILBERT");
     if (o->slot_block != 0 && o->slot_block != 16)
         return fail(SWE_HIP_EINVAL, "slot_block must be 0 or 16");
+    if (o->flags & ~(SWE_HIP_EXACT | SWE_HIP_FP32))
+        return fail(SWE_HIP_EINVAL, "unknown flag bits");
+    const bool fp32 = (o->flags & SWE_HIP_FP32) != 0;
+    if (fp32 && (o->members != 1 || o->tiled))
+        return fail(SWE_HIP_EINVAL, "SWE_HIP_FP32 contexts hold one state (members == 1, not "
+                                    "tiled)");
+    if (o->share && o->share->fp32 != fp32)
+        return fail(SWE_HIP_EINVAL, "share: the shared context has the other model precision");
     if (o->members == 1 && o->tile_nodes != 0 && o->tile_nodes != 64 && o->tile_nodes != 128
         && o->tile_nodes != 256)
         return fail(SWE_HIP_EINVAL, "tile_nodes of the single-state tile kernel must be 0, 64, "
@@ -723,7 +816,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         m = o->share->mesh;
         m->refs++;
     } else {
-        int rc = build_mesh(s, device, o->node_order, o->slot_block, &m);
+        int rc = build_mesh(s, device, o->node_order, o->slot_block,
+                            (o->flags & SWE_HIP_FP32) != 0, &m);
         if (rc)
             return rc;
     }
@@ -752,6 +846,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         c->kernel = SWE_HIP_KERNEL_QUADW;   // both need a compile-time stencil width
     else
         c->kernel = o->kernel;
+    c->fp32 = fp32;
     c->prefetch = o->prefetch;
     c->tile_nodes = o->tile_nodes;
     c->tile_rows = o->tile_rows;
@@ -767,20 +862,23 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     // F (the RHS array of the drop-in path) is allocated on first use: the
     // fused stages never store the RHS
-    const size_t sbytes = sizeof(double) * (size_t)state_count(c);
-    double **bufs[] = {&c->d_H, &c->d_Ka, &c->d_Kb, &c->d_D};
-    for (double **b : bufs)
-        if (!rc) rc = dev_alloc(&c->bytes, (void **)b, sbytes);
-    if (!rc) rc = dev_alloc(&c->bytes, (void **)&c->d_stage, sizeof(double) * 4 * (size_t)c->n);
+    const size_t sbytes = (fp32 ? sizeof(float) : sizeof(double)) * (size_t)state_count(c);
+    void **bufs[] = {(void **)&c->d_H, (void **)&c->d_Ka, (void **)&c->d_Kb, (void **)&c->d_D};
+    void **bufs32[] = {(void **)&c->f_H, (void **)&c->f_Ka, (void **)&c->f_Kb, (void **)&c->f_D};
+    for (int k = 0; k < 4; ++k)
+        if (!rc) rc = dev_alloc(&c->bytes, fp32 ? bufs32[k] : bufs[k], sbytes);
+    if (!rc && !fp32)
+        rc = dev_alloc(&c->bytes, (void **)&c->d_stage, sizeof(double) * 4 * (size_t)c->n);
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&c->d_active, sizeof(int));
-    for (double **b : bufs)
+    for (int k = 0; k < 4; ++k)
         if (!rc)
-            check(hipMemsetAsync(*b, 0, sbytes, c->stream), "hipMemsetAsync");
+            check(hipMemsetAsync(*(fp32 ? bufs32[k] : bufs[k]), 0, sbytes, c->stream),
+                  "hipMemsetAsync");
     if (!rc) {
         k_set_int<<<1, 64, 0, c->stream>>>(c->d_active, c->B);
         rc = check_launch("k_set_int");
     }
-    if (!rc && c->B == 1 && c->kernel == SWE_HIP_KERNEL_TILE) {
+    if (!rc && c->B == 1 && c->kernel == SWE_HIP_KERNEL_TILE && !fp32) {
         if (!c->tile_nodes)
             c->tile_nodes = kPtileNodes;
         rc = build_tiles(c, c->tile_nodes, 4 * c->tile_nodes);
@@ -825,6 +923,51 @@ int swe_hip_create(const swe_hip_static *s, int members, unsigned flags, int dev
 
 void swe_hip_destroy(swe_hip_ctx *ctx) { free_ctx(ctx); }
 
+int swe_hip_create32(const swe_hip_static32 *s, int members, unsigned flags, int device,
+                     swe_hip_ctx **out)
+{
+    if (!out)
+        return fail(SWE_HIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!s)
+        return fail(SWE_HIP_EINVAL, "static data is NULL");
+    const long n = s->compute_size, nw = n * (long)s->padded_Nnbr;
+    if (n <= 0 || s->padded_Nnbr <= 0 || !s->x || !s->y || !s->z || !s->f || !s->ghm || !s->p_u
+        || !s->p_v || !s->p_w || !s->gradghm || !s->idx || !s->Dx || !s->Dy || !s->Dz || !s->L)
+        return fail(SWE_HIP_EINVAL, "bad sizes or a NULL static array");
+    // widen to the double descriptor (exact), repacked and rounded back to
+    // float on upload by the FP32 context
+    std::vector<double> buf((size_t)(5 * n + 12 * n + 4 * nw));
+    double *p = buf.data();
+    auto widen = [&](const float *a, long count) {
+        double *dst = p;
+        for (long k = 0; k < count; ++k)
+            dst[k] = (double)a[k];
+        p += count;
+        return (const double *)dst;
+    };
+    swe_hip_static d{};
+    d.compute_size = s->compute_size;
+    d.patch_size = s->patch_size;
+    d.compute_pid_s = s->compute_pid_s;
+    d.Nnbr = s->Nnbr;
+    d.padded_Nnbr = s->padded_Nnbr;
+    d.cfdl = s->cfdl;
+    d.sfdl = s->sfdl;
+    d.simd = s->simd;
+    d.gh0 = s->gh0;
+    d.dt = s->dt;
+    d.x = widen(s->x, n); d.y = widen(s->y, n); d.z = widen(s->z, n);
+    d.f = widen(s->f, n); d.ghm = widen(s->ghm, n);
+    d.p_u = widen(s->p_u, 3 * n); d.p_v = widen(s->p_v, 3 * n); d.p_w = widen(s->p_w, 3 * n);
+    d.gradghm = widen(s->gradghm, 3 * n);
+    d.idx = s->idx;
+    d.Dx = widen(s->Dx, nw); d.Dy = widen(s->Dy, nw); d.Dz = widen(s->Dz, nw);
+    d.L = widen(s->L, nw);
+    swe_hip_options o{members, flags | SWE_HIP_FP32, device, 0, nullptr, 0, 0, 0, 0, 0, 0, 0};
+    return swe_hip_create_ex(&d, &o, out);
+}
+
 int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, int steps,
                               int *best_mb, double *best_ms)
 {
@@ -853,7 +996,10 @@ int swe_hip_info(const swe_hip_ctx *c, int *n_nodes, int *members, int *nnbr, un
     return 0;
 }
 
-void *swe_hip_state_device_ptr(swe_hip_ctx *c) { return c ? (void *)c->d_H : nullptr; }
+void *swe_hip_state_device_ptr(swe_hip_ctx *c)
+{
+    return c ? (c->fp32 ? (void *)c->f_H : (void *)c->d_H) : nullptr;
+}
 void *swe_hip_stream(swe_hip_ctx *c) { return c ? (void *)c->stream : nullptr; }
 long long swe_hip_device_bytes(const swe_hip_ctx *c) { return c ? c->bytes : 0; }
 long long swe_hip_static_bytes(const swe_hip_ctx *c) { return c ? c->mesh->bytes : 0; }
@@ -945,10 +1091,45 @@ static int from_device_member(swe_hip_ctx *c, int member, const double *src, dou
     return 0;
 }
 
+// FP32 single state: host layout (SVM_LIN_ID) <-> device [node][4] floats.
+static int to_device32(swe_hip_ctx *c, const float *Hhost, float *dst)
+{
+    const int n = c->n;
+    std::vector<float> t(4 * (size_t)n);
+    const std::vector<int> &perm = c->mesh->perm;
+    for (int d = 0; d < n; ++d) {
+        const int i = perm.empty() ? d : perm[d];
+        for (int v = 0; v < 4; ++v)
+            t[4 * (size_t)d + v] = Hhost[c->lay.svm(i, v)];
+    }
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpyAsync(dst, t.data(), sizeof(float) * 4 * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int from_device32(swe_hip_ctx *c, const float *src, float *Hhost)
+{
+    const int n = c->n;
+    std::vector<float> t(4 * (size_t)n);
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipMemcpyAsync(t.data(), src, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const std::vector<int> &perm = c->mesh->perm;
+    for (int d = 0; d < n; ++d) {
+        const int i = perm.empty() ? d : perm[d];
+        for (int v = 0; v < 4; ++v)
+            Hhost[c->lay.svm(i, v)] = t[4 * (size_t)d + v];
+    }
+    return 0;
+}
+
 int swe_hip_set_state(swe_hip_ctx *c, int member, const double *H)
 {
     if (!c || !H || member < 0 || member >= (c ? c->B : 0))
         return fail(SWE_HIP_EINVAL, "set_state: bad ctx, pointer or member");
+    if (c->fp32)
+        return fail(SWE_HIP_EINVAL, "set_state: FP32 context, use swe_hip_set_state32");
     return to_device_member(c, member, H, c->d_H);
 }
 
@@ -956,28 +1137,117 @@ int swe_hip_get_state(swe_hip_ctx *c, int member, double *H)
 {
     if (!c || !H || member < 0 || member >= (c ? c->B : 0))
         return fail(SWE_HIP_EINVAL, "get_state: bad ctx, pointer or member");
+    if (c->fp32)
+        return fail(SWE_HIP_EINVAL, "get_state: FP32 context, use swe_hip_get_state32");
     return from_device_member(c, member, c->d_H, H);
+}
+
+int swe_hip_set_state32(swe_hip_ctx *c, int member, const float *H)
+{
+    if (!c || !H || member != 0 || !c->fp32)
+        return fail(SWE_HIP_EINVAL, "set_state32: bad ctx, pointer or member (FP32 contexts "
+                                    "only, member 0)");
+    return to_device32(c, H, c->f_H);
+}
+
+int swe_hip_get_state32(swe_hip_ctx *c, int member, float *H)
+{
+    if (!c || !H || member != 0 || !c->fp32)
+        return fail(SWE_HIP_EINVAL, "get_state32: bad ctx, pointer or member (FP32 contexts "
+                                    "only, member 0)");
+    return from_device32(c, c->f_H, H);
 }
 
 // The RHS buffer F of the drop-in path, allocated (and zeroed) on its first
 // use.
 static int ensure_F(swe_hip_ctx *c)
 {
-    if (c->d_F)
+    void **F = c->fp32 ? (void **)&c->f_F : (void **)&c->d_F;
+    if (*F)
         return 0;
     HIPCHK(hipSetDevice(c->dev));
-    const size_t sbytes = sizeof(double) * (size_t)state_count(c);
-    int rc = dev_alloc(&c->bytes, (void **)&c->d_F, sbytes);
+    const size_t sbytes = (c->fp32 ? sizeof(float) : sizeof(double)) * (size_t)state_count(c);
+    int rc = dev_alloc(&c->bytes, F, sbytes);
     if (rc)
         return rc;
-    HIPCHK(hipMemsetAsync(c->d_F, 0, sbytes, c->stream));
+    HIPCHK(hipMemsetAsync(*F, 0, sbytes, c->stream));
     return 0;
+}
+
+static void launch_rhs32(const swe_hip_ctx *c, const float *src, float *F)
+{
+    DevStage32 st{};
+    st.src = src;
+    st.h = src;
+    st.out = F;
+    launch_fp32_stage(c, ST_RHS, st);
+}
+
+int swe_hip_eval_rhs32(swe_hip_ctx *c, int member, const float *K, float *F)
+{
+    if (!c || !K || !F || member != 0 || !c->fp32)
+        return fail(SWE_HIP_EINVAL, "eval_rhs32: bad ctx, pointer or member (FP32 contexts "
+                                    "only, member 0)");
+    int rc = ensure_F(c);
+    if (!rc)
+        rc = to_device32(c, K, c->f_Kb);
+    if (rc)
+        return rc;
+    launch_rhs32(c, c->f_Kb, c->f_F);
+    rc = check_launch("eval_RHS kernel (FP32)");
+    if (rc)
+        return rc;
+    return from_device32(c, c->f_F, F);
+}
+
+// The drop-in substep of an FP32 context: RK_substep with the reference's
+// FP32 types (rk4_rbffd_swe.c:30-186; eval_K's coefficient dt/2.0 rounded
+// to float, update_H's dt/6.0 kept in double).
+static int rk_substep32(swe_hip_ctx *c, int s)
+{
+    const long cnt = state_count(c);
+    const int eb = elementwise_blocks(cnt);
+    const float dt = (float)c->dt;
+    const bool ex = exact(c);
+    float *H = c->f_H, *K = c->f_Ka, *F = c->f_F, *D = c->f_D;
+    hipStream_t q = c->stream;
+    int rc = timed(c, &c->acc.t_eval_rhs, "eval_RHS kernel (FP32)",
+                   [&] { launch_rhs32(c, s == 0 ? H : K, F); });
+    if (rc)
+        return rc;
+    if (s == 0) {
+        rc = timed(c, nullptr, "copy kernel", [&] { k_copy32<<<eb, kBlock, 0, q>>>(D, F, cnt); });
+    } else {
+        const float coeff = s == 3 ? 1.0f : 2.0f;
+        rc = timed(c, &c->acc.t_update_D, "update_D kernel", [&] {
+            if (ex) k_axpy_inplace32<true><<<eb, kBlock, 0, q>>>(D, coeff, F, cnt);
+            else k_axpy_inplace32<false><<<eb, kBlock, 0, q>>>(D, coeff, F, cnt);
+        });
+    }
+    if (rc)
+        return rc;
+    if (s < 3) {
+        const float cK = s == 2 ? dt : (float)(dt / 2.0);
+        rc = timed(c, &c->acc.t_eval_K, "eval_K kernel", [&] {
+            if (ex) k_axpy_to32<true><<<eb, kBlock, 0, q>>>(K, H, cK, F, cnt);
+            else k_axpy_to32<false><<<eb, kBlock, 0, q>>>(K, H, cK, F, cnt);
+        });
+    } else {
+        const double c6 = dt / 6.0;
+        rc = timed(c, &c->acc.t_update_H, "update_H kernel", [&] {
+            if (ex) k_update_h32<true><<<eb, kBlock, 0, q>>>(H, c6, D, cnt);
+            else k_update_h32<false><<<eb, kBlock, 0, q>>>(H, c6, D, cnt);
+        });
+    }
+    return rc;
 }
 
 int swe_hip_eval_rhs(swe_hip_ctx *c, int member, const double *K, double *F)
 {
     if (!c || !K || !F || member < 0 || member >= (c ? c->B : 0))
         return fail(SWE_HIP_EINVAL, "eval_rhs: bad ctx, pointer or member");
+    if (c->fp32)
+        return fail(SWE_HIP_EINVAL, "eval_rhs: FP32 context, use swe_hip_eval_rhs32");
     int rc = ensure_F(c);
     if (rc)
         return rc;
@@ -998,6 +1268,8 @@ int swe_hip_rk_substep(swe_hip_ctx *c, int s)
     HIPCHK(hipSetDevice(c->dev));
     if (int rc0 = ensure_F(c))
         return rc0;
+    if (c->fp32)
+        return rk_substep32(c, s);
     const long cnt = state_count(c);
     const int eb = elementwise_blocks(cnt);
     const double dt = c->dt;

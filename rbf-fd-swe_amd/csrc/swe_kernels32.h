@@ -1,0 +1,185 @@
+// FP32 stage kernels: the reference's shipped model precision (RUN_32BIT,
+// fType = float, swe_code/main/include/swe_config.h:75-80).
+//
+// Device layouts as the FP64 path (swe_kernels.h) with float values:
+//   wts32 f32 [nnbr][ld][4] (Dx, Dy, Dz, L of one slot and node: 16 B),
+//   geo32 f32 [GEO_N][ld], state f32 [node][u v w h] (16 B per node).
+// One lane per node: per stencil slot one 16-byte weight load, one 16-byte
+// neighbour-state gather and one id load, the 16 sums in float.
+//
+// Arithmetic follows the reference's C types (the build uses
+// -ffp-contract=off; EXACT kernels round every float product and sum
+// separately, in the reference's order, and reproduce its FP32 build bit for
+// bit): the stencil sums and the pointwise RHS are float
+// (rk4_rbffd_swe.c:254-353); eval_K's coefficient dt/2.0 or dt is a float
+// argument and K = H + c*F is float (:123-158); update_D is float
+// (:160-186); update_H computes (dt/6.0)*D and the sum with H in double and
+// rounds once to float on the store (:86-121: dt/6.0 is a double
+// expression).  The fast kernels use fused multiply-adds.
+#pragma once
+
+#include "swe_kernels.h"
+
+namespace swe {
+
+struct DevMesh32 {
+    const int *__restrict__ nbr;
+    const float *__restrict__ wts;
+    const float *__restrict__ geo;
+    int n, ld, nnbr, sblk;
+    float gh0;
+};
+
+struct DevStage32 {
+    const float *src;   // gather source: H (stage 0) or K
+    const float *h;     // H, own node
+    float *d;           // D
+    float *out;         // next K, H (stage 3) or F (ST_RHS)
+    float c;            // K = H + c*F (float dt/2.0, dt/2.0, dt)
+    double c6;          // stage 3: H += (dt/6.0)*D in double
+    float dcoef;        // D += dcoef*F (2, 2, 1)
+    int nblk;
+};
+
+template <bool EXACT>
+__device__ __forceinline__ float madd32(float acc, float w, float s)
+{
+    if constexpr (EXACT) {
+        const float p = w * s;
+        return acc + p;
+    } else {
+        return __builtin_fmaf(w, s, acc);
+    }
+}
+
+// y + (a*x) in float.
+template <bool EXACT>
+__device__ __forceinline__ float axpy32(float y, float a, float x)
+{
+    if constexpr (EXACT) {
+        const float p = a * x;
+        return y + p;
+    } else {
+        return __builtin_fmaf(a, x, y);
+    }
+}
+
+// update_H: (float)((double)H + (dt/6.0) * (double)D).
+template <bool EXACT>
+__device__ __forceinline__ float update_h32(float h, double c6, float d)
+{
+    if constexpr (EXACT) {
+        const double p = c6 * (double)d;
+        return (float)((double)h + p);
+    } else {
+        return (float)__builtin_fma(c6, (double)d, (double)h);
+    }
+}
+
+struct Sums32 {
+    float dudx, dvdx, dwdx, dhdx;
+    float dudy, dvdy, dwdy, dhdy;
+    float dudz, dvdz, dwdz, dhdz;
+    float lu, lv, lw, lh;
+};
+
+// Pointwise right-hand side of one node in float, operation order of
+// rk4_rbffd_swe.c:331-353.
+__device__ __forceinline__ void rhs_point32(const Sums32 &S, float u, float v, float w, float h,
+                                            const float *g, float gh0, float F[4])
+{
+    const float x = g[G_X], y = g[G_Y], z = g[G_Z], f = g[G_F], ghm = g[G_GHM];
+    const float ru = -((u * S.dudx) + (v * S.dudy) + (w * S.dudz) + (f * ((y * w) - (z * v)))
+                       + S.dhdx);
+    const float rv = -((u * S.dvdx) + (v * S.dvdy) + (w * S.dvdz) + (f * ((z * u) - (x * w)))
+                       + S.dhdy);
+    const float rw = -((u * S.dwdx) + (v * S.dwdy) + (w * S.dwdz) + (f * ((x * v) - (y * u)))
+                       + S.dhdz);
+    F[0] = (g[G_PU] * ru) + (g[G_PU + 1] * rv) + (g[G_PU + 2] * rw) + S.lu;
+    F[1] = (g[G_PV] * ru) + (g[G_PV + 1] * rv) + (g[G_PV + 2] * rw) + S.lv;
+    F[2] = (g[G_PW] * ru) + (g[G_PW + 1] * rv) + (g[G_PW + 2] * rw) + S.lw;
+    F[3] = -((u * (S.dhdx - g[G_GG])) + (v * (S.dhdy - g[G_GG + 1]))
+             + (w * (S.dhdz - g[G_GG + 2])) + ((h + gh0 - ghm) * (S.dudx + S.dvdy + S.dwdz)))
+           + S.lh;
+}
+
+// One node's fused stage update of its four values (16-byte accesses).
+template <int STAGE, bool EXACT>
+__device__ __forceinline__ void stage_update32(const DevStage32 &st, long e, const float F[4])
+{
+    float4 *out = reinterpret_cast<float4 *>(st.out + e);
+    if constexpr (STAGE == ST_RHS) {
+        *out = make_float4(F[0], F[1], F[2], F[3]);
+    } else {
+        const float4 hv = *reinterpret_cast<const float4 *>(st.h + e);
+        float4 *dp = reinterpret_cast<float4 *>(st.d + e);
+        if constexpr (STAGE == ST_0) {
+            *dp = make_float4(F[0], F[1], F[2], F[3]);
+            *out = make_float4(axpy32<EXACT>(hv.x, st.c, F[0]), axpy32<EXACT>(hv.y, st.c, F[1]),
+                               axpy32<EXACT>(hv.z, st.c, F[2]), axpy32<EXACT>(hv.w, st.c, F[3]));
+        } else {
+            const float4 d0 = *dp;
+            const float4 dv = make_float4(axpy32<EXACT>(d0.x, st.dcoef, F[0]),
+                                          axpy32<EXACT>(d0.y, st.dcoef, F[1]),
+                                          axpy32<EXACT>(d0.z, st.dcoef, F[2]),
+                                          axpy32<EXACT>(d0.w, st.dcoef, F[3]));
+            if constexpr (STAGE == ST_3) {
+                *out = make_float4(update_h32<EXACT>(hv.x, st.c6, dv.x),
+                                   update_h32<EXACT>(hv.y, st.c6, dv.y),
+                                   update_h32<EXACT>(hv.z, st.c6, dv.z),
+                                   update_h32<EXACT>(hv.w, st.c6, dv.w));
+            } else {
+                *dp = dv;
+                *out = make_float4(axpy32<EXACT>(hv.x, st.c, F[0]), axpy32<EXACT>(hv.y, st.c, F[1]),
+                                   axpy32<EXACT>(hv.z, st.c, F[2]), axpy32<EXACT>(hv.w, st.c, F[3]));
+            }
+        }
+    }
+}
+
+// Single state, one lane per node, FP32.  NNBR > 0: compile-time stencil
+// width; 0: run-time width.
+template <int STAGE, bool EXACT, int NNBR>
+__global__ __launch_bounds__(256) void k_stage_lane32(DevMesh32 m, DevStage32 st)
+{
+    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
+    const bool live = t < m.n;
+    const int node = live ? t : m.n - 1;
+    const int nn = NNBR > 0 ? NNBR : m.nnbr;
+    const long ld = m.sblk ? 16L : (long)m.ld;
+    const long W = NNBR > 0 ? NNBR : m.nnbr;
+    const long sb = m.sblk ? ((long)(node >> 4) * W << 4) + (node & 15) : (long)node;
+    const int *__restrict__ nb = m.nbr + sb;
+    const float *__restrict__ wp = m.wts + 4L * sb;
+    const float *__restrict__ src = st.src;
+    Sums32 S = {};
+    auto slot = [&](int j) {
+        const int q = nb[j * ld];
+        const float4 w = *reinterpret_cast<const float4 *>(wp + 4L * j * ld);
+        const float4 s = *reinterpret_cast<const float4 *>(src + 4L * q);
+        S.dudx = madd32<EXACT>(S.dudx, w.x, s.x); S.dvdx = madd32<EXACT>(S.dvdx, w.x, s.y);
+        S.dwdx = madd32<EXACT>(S.dwdx, w.x, s.z); S.dhdx = madd32<EXACT>(S.dhdx, w.x, s.w);
+        S.dudy = madd32<EXACT>(S.dudy, w.y, s.x); S.dvdy = madd32<EXACT>(S.dvdy, w.y, s.y);
+        S.dwdy = madd32<EXACT>(S.dwdy, w.y, s.z); S.dhdy = madd32<EXACT>(S.dhdy, w.y, s.w);
+        S.dudz = madd32<EXACT>(S.dudz, w.z, s.x); S.dvdz = madd32<EXACT>(S.dvdz, w.z, s.y);
+        S.dwdz = madd32<EXACT>(S.dwdz, w.z, s.z); S.dhdz = madd32<EXACT>(S.dhdz, w.z, s.w);
+        S.lu = madd32<EXACT>(S.lu, w.w, s.x); S.lv = madd32<EXACT>(S.lv, w.w, s.y);
+        S.lw = madd32<EXACT>(S.lw, w.w, s.z); S.lh = madd32<EXACT>(S.lh, w.w, s.w);
+    };
+    // a rolled loop of 4-slot bodies keeps ~4 slots of loads in flight per
+    // lane (a fully unrolled one front-loads all 31 and spills)
+#pragma unroll 4
+    for (int j = 0; j < nn; ++j)
+        slot(j);
+    const float4 own = *reinterpret_cast<const float4 *>(src + 4L * node);
+    float g[GEO_N];
+#pragma unroll
+    for (int f = 0; f < GEO_N; ++f)
+        g[f] = m.geo[(long)f * m.ld + node];
+    float F[4];
+    rhs_point32(S, own.x, own.y, own.z, own.w, g, m.gh0, F);
+    if (live)
+        stage_update32<STAGE, EXACT>(st, 4L * node, F);
+}
+
+}  // namespace swe

@@ -70,8 +70,8 @@ extern "C" int swe_hip_rk_steps_noalloc(swe_hip_ctx *c, double *H, int nsteps)
     using namespace swe;
     if (!c || !H || nsteps < 0)
         return set_error(SWE_HIP_EINVAL, "rk_steps_noalloc: bad ctx, state or nsteps");
-    if (c->B != 1)
-        return set_error(SWE_HIP_EINVAL, "rk_steps_noalloc: single-state contexts only "
+    if (c->B != 1 || c->fp32)
+        return set_error(SWE_HIP_EINVAL, "rk_steps_noalloc: single-state FP64 contexts only "
                                          "(members == 1)");
     if ((long)c->n > kNoallocMaxNodes)
         return set_error(

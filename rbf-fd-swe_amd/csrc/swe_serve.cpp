@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
+#include <cstdio>
 #include <deque>
 #include <string>
 #include <thread>
@@ -44,6 +46,8 @@ struct Batch {
 
 struct Mesh {
     swe_hip_static desc{};
+    std::vector<int> mapping;    // host node -> file node (empty: identity)
+    int n_nodes = 0;             // real nodes (result files)
     long long static_est = 0;    // bytes of the device static data
     long long slot_state_est = 0;
     long long batch_extra_est = 0;   // per batch context: tile lists and node-major copies
@@ -60,6 +64,7 @@ struct Request {
     int nsteps = 0;
     std::vector<const double *> init;
     std::vector<double *> result;
+    std::string result_path;     // written at completion when not empty
     int started = 0;
     int done = 0;
     double t_submit = 0.0, t_admit = -1.0;
@@ -91,6 +96,40 @@ void charge(swe_srv *s, long long bytes)
     s->hbm_peak = std::max(s->hbm_peak, s->hbm_in_use);
 }
 
+// The host-layout index of (node, variable) (SVM_LIN_ID, swe_config.h:50-54).
+long svm_index(const swe_hip_static &d, int node, int v)
+{
+    const int S = d.simd;
+    return d.cfdl ? 4L * node + v : (long)(node / S) * S * 4 + (long)v * S + node % S;
+}
+
+// A finished request's result file, in the history format of swe_run: int32
+// N, then one record per member: int32 step (= nsteps), f64 time, f64
+// H[N][4] (u, v, w, h) in the file's node order.
+int write_result(const Mesh &m, const Request &r)
+{
+    FILE *fp = fopen(r.result_path.c_str(), "wb");
+    if (!fp)
+        return SWE_SRV_EIO;
+    const int N = m.n_nodes;
+    std::vector<double> row(4 * (size_t)N);
+    const int32_t n32 = N, st = r.nsteps;
+    const double t = r.nsteps * m.desc.dt;
+    bool ok = fwrite(&n32, sizeof(n32), 1, fp) == 1;
+    for (int k = 0; ok && k < r.members; ++k) {
+        const double *H = r.result[k];
+        for (int i = 0; i < N; ++i) {
+            const int fn = m.mapping.empty() ? i : m.mapping[i];
+            for (int v = 0; v < 4; ++v)
+                row[4 * (size_t)fn + v] = H[svm_index(m.desc, i, v)];
+        }
+        ok = fwrite(&st, sizeof(st), 1, fp) == 1 && fwrite(&t, sizeof(t), 1, fp) == 1
+             && fwrite(row.data(), sizeof(double), row.size(), fp) == row.size();
+    }
+    ok = (fclose(fp) == 0) && ok;
+    return ok ? 0 : SWE_SRV_EIO;
+}
+
 // Reads back and releases members whose budget is spent (batch idle).
 int retire(swe_srv *s, Batch &b, int *progress)
 {
@@ -105,6 +144,11 @@ int retire(swe_srv *s, Batch &b, int *progress)
         if (rc)
             return rc;
         if (++r.done == r.members) {
+            if (!r.result_path.empty()) {
+                rc = write_result(s->meshes[r.mesh], r);
+                if (rc)
+                    return rc;
+            }
             const double t = now_s();
             s->done.push_back({r.tag, t - r.t_submit, r.t_admit - r.t_submit});
             s->requests_done++;
@@ -322,10 +366,24 @@ void swe_srv_destroy(swe_srv *s)
 
 int swe_srv_add_mesh(swe_srv *s, const swe_hip_static *d, int *mesh_id)
 {
-    if (!s || !d || !mesh_id || d->compute_size <= 0 || d->Nnbr <= 0)
+    return swe_srv_add_mesh_ex(s, d, nullptr, d ? d->compute_size : 0, mesh_id);
+}
+
+int swe_srv_add_mesh_ex(swe_srv *s, const swe_hip_static *d, const int *mapping, int n_nodes,
+                        int *mesh_id)
+{
+    if (!s || !d || !mesh_id || d->compute_size <= 0 || d->Nnbr <= 0 || n_nodes < 1
+        || n_nodes > d->compute_size)
         return SWE_HIP_EINVAL;
     Mesh m;
     m.desc = *d;
+    m.n_nodes = n_nodes;
+    if (mapping) {
+        m.mapping.assign(mapping, mapping + n_nodes);
+        for (int v : m.mapping)
+            if (v < 0 || v >= n_nodes)
+                return SWE_HIP_EINVAL;
+    }
     const long long n = d->compute_size;
     m.static_est = n * ((long long)d->Nnbr * (4 + 32) + 17 * 8);
     m.slot_state_est = n * 4 * 8 * 4;   // H, K (two), D per member slot
@@ -341,10 +399,19 @@ int swe_srv_add_mesh(swe_srv *s, const swe_hip_static *d, int *mesh_id)
 int swe_srv_submit(swe_srv *s, int mesh_id, int members, const double *const *init,
                    double *const *result, int nsteps, long long tag, int *req_id)
 {
+    return swe_srv_submit_ex(s, mesh_id, members, init, result, nsteps, tag, nullptr, req_id);
+}
+
+int swe_srv_submit_ex(swe_srv *s, int mesh_id, int members, const double *const *init,
+                      double *const *result, int nsteps, long long tag, const char *result_path,
+                      int *req_id)
+{
     if (!s || mesh_id < 0 || mesh_id >= (int)s->meshes.size() || members < 1 || !init
         || !result || nsteps < 0 || !req_id)
         return SWE_HIP_EINVAL;
     Request r;
+    if (result_path)
+        r.result_path = result_path;
     r.tag = tag;
     r.mesh = mesh_id;
     r.members = members;

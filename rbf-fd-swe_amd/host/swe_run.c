@@ -18,6 +18,11 @@
  *                    attempt, in the file node order (RCM and padding undone):
  *                    int32 N, then per snapshot int32 step, f64 time,
  *                    f64 H[N][4] (u, v, w, h)
+ *   SWE_RESTART_FILE, SWE_RESTART_STEP   restart: every attempt starts from
+ *                    the snapshot of that step (default: the file's last) of
+ *                    a history file, the run ends at SWE_NSTEPS as if it had
+ *                    never stopped (padding rows repeat the last node, as the
+ *                    reference pads H_init, matrix_transformations.c:20)
  * Init pipeline main.c:94-190, attempt loop :294-392, verify_output
  * :439-477 (every member against H_100), profiling report profiling.c.
  * Errors: a missing parameter or a NetCDF request prints and exits 1, an
@@ -68,6 +73,50 @@ static int write_snapshot(FILE *fp, const swe_problem *prob, const double *H, in
     return ok ? 0 : -1;
 }
 
+/* Snapshot `step` (< 0: the last) of a history file -> host layout H (the
+ * inverse of write_snapshot; padding rows copy the last node).  Returns 0,
+ * or -1 when the file does not match the problem or has no such step. */
+static int read_snapshot(const char *path, const swe_problem *prob, int step, double *H,
+                         int *got_step)
+{
+    const int N = prob->g.Nnodes;
+    FILE *fp = fopen(path, "rb");
+    if (!fp)
+        return -1;
+    int32_t n32 = 0;
+    const size_t rec = sizeof(int32_t) + sizeof(double) + sizeof(double) * 4 * (size_t)N;
+    double *row = (double *)malloc(sizeof(double) * 4 * (size_t)N);
+    int found = 0;
+    if (row && fread(&n32, sizeof(n32), 1, fp) == 1 && n32 == N) {
+        int32_t st;
+        double t;
+        while (fread(&st, sizeof(st), 1, fp) == 1 && fread(&t, sizeof(t), 1, fp) == 1) {
+            if (step >= 0 && st != step) {
+                if (fseek(fp, (long)(rec - sizeof(st) - sizeof(t)), SEEK_CUR) != 0)
+                    break;
+                continue;
+            }
+            if (fread(row, sizeof(double), 4 * (size_t)N, fp) != 4 * (size_t)N)
+                break;
+            found = 1;
+            *got_step = st;
+            if (step >= 0)
+                break;
+        }
+    }
+    fclose(fp);
+    if (found) {
+        for (int i = 0; i < prob->patch.patch_size; i++) {
+            const int node = i < N ? i : N - 1;
+            const int file_node = prob->mapping ? prob->mapping[node] : node;
+            for (int v = 0; v < 4; v++)
+                H[swe_svm_lin_id(&prob->layout, i, v)] = row[4 * (size_t)file_node + v];
+        }
+    }
+    free(row);
+    return found ? 0 : -1;
+}
+
 static void hip_check(int rc, const char *what)
 {
     if (rc != SWE_HIP_OK) {
@@ -91,6 +140,8 @@ int main(void)
     const int members = env_int("SWE_MEMBERS", 1);
     const char *hist_path = getenv("SWE_OUTPUT_FILE");
     const int hist_int = env_int("SWE_HIST_INT", 0);
+    const char *restart_path = getenv("SWE_RESTART_FILE");
+    const int restart_want = env_int("SWE_RESTART_STEP", -1);
     swe_layout lay = {0, 0, env_int("SWE_SIMD_LENGTH", 4)};
     if (!strcmp(layout_name, "cfdl")) lay.cfdl = 1;
     else if (!strcmp(layout_name, "sfdl")) lay.sfdl = 1;
@@ -145,6 +196,23 @@ int main(void)
     double *H = (double *)malloc(sizeof(double) * 4 * (size_t)p->patch_size);
     if (!H)
         return 1;
+    /* the initial state of every attempt: H_init, or a restart snapshot */
+    const double *H_start = prob.H_init;
+    double *H_restart = NULL;
+    int start_step = 0;
+    if (restart_path) {
+        H_restart = (double *)malloc(sizeof(double) * 4 * (size_t)p->patch_size);
+        if (!H_restart || read_snapshot(restart_path, &prob, restart_want, H_restart, &start_step)) {
+            printf("Error reading restart snapshot %d of %s\n", restart_want, restart_path);
+            return 2;
+        }
+        if (start_step > nsteps) {
+            fprintf(stderr, "swe_run: restart step %d beyond SWE_NSTEPS %d\n", start_step, nsteps);
+            return 1;
+        }
+        H_start = H_restart;
+        printf("Restart from step %d of %s\n", start_step, restart_path);
+    }
     FILE *hist = NULL;
     if (hist_path && hist_int > 0) {
         hist = fopen(hist_path, "wb");
@@ -159,13 +227,15 @@ int main(void)
     }
     for (int attempt = 0; attempt < nattempts; attempt++) {
         for (int m = 0; m < members; m++)
-            hip_check(swe_hip_set_state(ctx, m, prob.H_init), "swe_hip_set_state");
+            hip_check(swe_hip_set_state(ctx, m, H_start), "swe_hip_set_state");
         hip_check(swe_hip_set_timing(ctx, substep_mode), "swe_hip_set_timing");
         timer.attempt = attempt;
         const double t0 = swe_get_time();
         const int chunk = (hist && attempt == 0) ? hist_int : nsteps;
-        for (int done = 0; done < nsteps;) {
-            const int k = nsteps - done < chunk ? nsteps - done : chunk;
+        for (int done = start_step; done < nsteps;) {
+            /* history snapshots stay on multiples of SWE_HIST_INT */
+            const int to_mark = chunk - done % chunk;
+            const int k = nsteps - done < to_mark ? nsteps - done : to_mark;
             if (substep_mode) {
                 for (int i = 0; i < k; i++)
                     for (int rk = 0; rk < 4; rk++)
@@ -207,7 +277,8 @@ int main(void)
     }
     if (hist)
         fclose(hist);
-    swe_process_profiling(&timer, nsteps, 1, NULL);
+    free(H_restart);
+    swe_process_profiling(&timer, nsteps - start_step, 1, NULL);
     free(H);
     swe_hip_destroy(ctx);
     swe_problem_free(&prob);

@@ -15,6 +15,8 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 SWE_HIP_EXACT = 1
+SWE_HIP_FP32 = 2
+_fp = C.POINTER(C.c_float)
 KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4, "tile": 5}
 ORDERS = {"host": 0, "morton": 1, "hilbert": 2}
 
@@ -28,6 +30,7 @@ EXPORTS = (
     "swe_hip_set_active", "swe_hip_move_member", "swe_hip_static_bytes",
     "swe_hip_tune_member_block",
     "swe_hip_tune_ensemble", "swe_hip_rk_steps_noalloc",
+    "swe_hip_create32", "swe_hip_set_state32", "swe_hip_get_state32", "swe_hip_eval_rhs32",
 )
 
 
@@ -39,6 +42,16 @@ class SweHipStatic(C.Structure):
                 ("x", _dp), ("y", _dp), ("z", _dp), ("f", _dp), ("ghm", _dp),
                 ("p_u", _dp), ("p_v", _dp), ("p_w", _dp), ("gradghm", _dp),
                 ("idx", _ip), ("Dx", _dp), ("Dy", _dp), ("Dz", _dp), ("L", _dp)]
+
+
+class SweHipStatic32(C.Structure):
+    _fields_ = [("compute_size", C.c_int), ("patch_size", C.c_int),
+                ("compute_pid_s", C.c_int), ("Nnbr", C.c_int), ("padded_Nnbr", C.c_int),
+                ("cfdl", C.c_int), ("sfdl", C.c_int), ("simd", C.c_int),
+                ("gh0", C.c_float), ("dt", C.c_float),
+                ("x", _fp), ("y", _fp), ("z", _fp), ("f", _fp), ("ghm", _fp),
+                ("p_u", _fp), ("p_v", _fp), ("p_w", _fp), ("gradghm", _fp),
+                ("idx", _ip), ("Dx", _fp), ("Dy", _fp), ("Dz", _fp), ("L", _fp)]
 
 
 class SweHipOptions(C.Structure):
@@ -88,6 +101,11 @@ def lib() -> C.CDLL:
         L.swe_hip_rk_substep.argtypes = [vp, C.c_int]
         L.swe_hip_rk_steps.argtypes = [vp, C.c_int]
         L.swe_hip_rk_steps_noalloc.argtypes = [vp, vp, C.c_int]
+        L.swe_hip_create32.argtypes = [C.POINTER(SweHipStatic32), C.c_int, C.c_uint, C.c_int,
+                                       C.POINTER(vp)]
+        L.swe_hip_set_state32.argtypes = [vp, C.c_int, _fp]
+        L.swe_hip_get_state32.argtypes = [vp, C.c_int, _fp]
+        L.swe_hip_eval_rhs32.argtypes = [vp, C.c_int, _fp, _fp]
         L.swe_hip_eval_rhs.argtypes = [vp, C.c_int, _dp, _dp]
         L.swe_hip_set_timing.argtypes = [vp, C.c_int]
         L.swe_hip_get_timers.argtypes = [vp, C.POINTER(SweHipTimers), C.c_int]
@@ -141,17 +159,53 @@ def static_from_problem(prob) -> tuple[SweHipStatic, list]:
     return s, keep
 
 
+def static32_from_problem(prob) -> tuple[SweHipStatic32, list]:
+    """swe_hip_static32 over a host.Problem read with fp32=True (its arrays
+    are floats held as doubles; converted exactly)."""
+    keep = []
+
+    def p(a, dtype=np.float32, t=_fp):
+        a32 = np.ascontiguousarray(np.asarray(a).astype(dtype))
+        if dtype == np.float32 and not np.array_equal(a32.astype(np.float64), np.asarray(a)):
+            raise ValueError("FP32 context: static data are not FP32-valued "
+                             "(read the problem with fp32=True)")
+        keep.append(a32)
+        return a32.ctypes.data_as(t)
+    s = SweHipStatic32()
+    s.compute_size, s.patch_size = prob.compute_size, prob.patch_size
+    s.compute_pid_s = prob.compute_pid_s
+    s.Nnbr, s.padded_Nnbr = prob.nnbr, prob.padded_nnbr
+    s.cfdl, s.sfdl, s.simd = prob.layout.cfdl, prob.layout.sfdl, prob.layout.simd
+    s.gh0, s.dt = prob.gh0, prob.dt
+    for nm in ("x", "y", "z", "f", "ghm", "p_u", "p_v", "p_w", "gradghm", "Dx", "Dy", "Dz", "L"):
+        setattr(s, nm, p(getattr(prob, nm)))
+    s.idx = p(prob.idx, np.int32, _ip)
+    return s, keep
+
+
 class HipContext:
-    """One device context: static data of one mesh + `members` state slots."""
+    """One device context: static data of one mesh + `members` state slots.
+    fp32=True: the reference's FP32 model type (states are float32 arrays)."""
 
     def __init__(self, prob, members: int = 1, exact: bool = False, device: int = 0,
                  member_block: int = 0, share: "HipContext | None" = None,
                  kernel: str = "auto", prefetch: int = 0, order: str = "host",
                  tiled: int = 0, slot_block: int = 0, tile_nodes: int = 0,
-                 tile_rows: int = 0):
+                 tile_rows: int = 0, fp32: bool = False):
         self.prob = prob
         self.members = int(members)
         self.exact = bool(exact)
+        self.fp32 = bool(fp32)
+        self.n_state = 4 * prob.patch_size
+        if self.fp32:
+            s, keep = static32_from_problem(prob)
+            h = C.c_void_p()
+            _chk(lib().swe_hip_create32(C.byref(s), self.members,
+                                        SWE_HIP_EXACT if exact else 0, device, C.byref(h)),
+                 "swe_hip_create32")
+            self._share = None
+            self._h = h
+            return
         opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
                              int(member_block), share.handle if share is not None else None,
                              KERNELS[kernel], int(prefetch), ORDERS[order], int(tiled),
@@ -165,7 +219,6 @@ class HipContext:
         _chk(rc, "swe_hip_create_ex")
         self._share = share            # keeps the static data's owner alive
         self._h = h
-        self.n_state = 4 * prob.patch_size
 
     @property
     def handle(self) -> C.c_void_p:
@@ -174,13 +227,25 @@ class HipContext:
         return self._h
 
     def set_state(self, H: np.ndarray, member: int = 0) -> None:
-        H = np.ascontiguousarray(H, dtype=np.float64)
+        if self.fp32:
+            H = np.ascontiguousarray(H, dtype=np.float32)
+        else:
+            H = np.ascontiguousarray(H, dtype=np.float64)
         if H.size != self.n_state:
             raise ValueError(f"state has {H.size} values, expected {self.n_state}")
-        _chk(lib().swe_hip_set_state(self.handle, member, H.ctypes.data_as(_dp)),
-             "swe_hip_set_state")
+        if self.fp32:
+            _chk(lib().swe_hip_set_state32(self.handle, member, H.ctypes.data_as(_fp)),
+                 "swe_hip_set_state32")
+        else:
+            _chk(lib().swe_hip_set_state(self.handle, member, H.ctypes.data_as(_dp)),
+                 "swe_hip_set_state")
 
     def get_state(self, member: int = 0) -> np.ndarray:
+        if self.fp32:
+            H = np.empty(self.n_state, dtype=np.float32)
+            _chk(lib().swe_hip_get_state32(self.handle, member, H.ctypes.data_as(_fp)),
+                 "swe_hip_get_state32")
+            return H
         H = np.empty(self.n_state)
         _chk(lib().swe_hip_get_state(self.handle, member, H.ctypes.data_as(_dp)),
              "swe_hip_get_state")
@@ -199,6 +264,12 @@ class HipContext:
              "swe_hip_rk_steps_noalloc")
 
     def eval_rhs(self, K: np.ndarray, member: int = 0) -> np.ndarray:
+        if self.fp32:
+            K = np.ascontiguousarray(K, dtype=np.float32)
+            F = np.empty(4 * self.prob.compute_size, dtype=np.float32)
+            _chk(lib().swe_hip_eval_rhs32(self.handle, member, K.ctypes.data_as(_fp),
+                                          F.ctypes.data_as(_fp)), "swe_hip_eval_rhs32")
+            return F
         K = np.ascontiguousarray(K, dtype=np.float64)
         F = np.empty(4 * self.prob.compute_size)
         _chk(lib().swe_hip_eval_rhs(self.handle, member, K.ctypes.data_as(_dp),

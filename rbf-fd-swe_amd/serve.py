@@ -13,10 +13,11 @@ from . import runtime
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
 
 EXPORTS = ("swe_srv_default_config", "swe_srv_create", "swe_srv_destroy", "swe_srv_add_mesh",
            "swe_srv_submit", "swe_srv_pump", "swe_srv_wait", "swe_srv_completed",
-           "swe_srv_get_stats")
+           "swe_srv_get_stats", "swe_srv_add_mesh_ex", "swe_srv_submit_ex")
 
 
 class SrvConfig(C.Structure):
@@ -57,6 +58,10 @@ def lib() -> C.CDLL:
         L.swe_srv_add_mesh.argtypes = [vp, C.POINTER(runtime.SweHipStatic), C.POINTER(C.c_int)]
         L.swe_srv_submit.argtypes = [vp, C.c_int, C.c_int, C.POINTER(_dp), C.POINTER(_dp),
                                      C.c_int, C.c_longlong, C.POINTER(C.c_int)]
+        L.swe_srv_add_mesh_ex.argtypes = [vp, C.POINTER(runtime.SweHipStatic), _ip, C.c_int,
+                                          C.POINTER(C.c_int)]
+        L.swe_srv_submit_ex.argtypes = [vp, C.c_int, C.c_int, C.POINTER(_dp), C.POINTER(_dp),
+                                        C.c_int, C.c_longlong, C.c_char_p, C.POINTER(C.c_int)]
         L.swe_srv_pump.argtypes = [vp, C.POINTER(C.c_int)]
         L.swe_srv_wait.argtypes = [vp]
         L.swe_srv_completed.argtypes = [vp, C.POINTER(C.c_longlong), _dp, _dp, C.c_int,
@@ -82,6 +87,7 @@ class Request:
     results: list = field(default_factory=list)
     latency_s: float = float("nan")
     queue_s: float = float("nan")
+    result_path: str | None = None   # result file written by the server at completion
 
 
 class Server:
@@ -107,9 +113,16 @@ class Server:
 
     def add_mesh(self, prob) -> int:
         s, keep = runtime.static_from_problem(prob)
+        mapping = None
+        if prob.mapping is not None:
+            mapping = np.ascontiguousarray(prob.mapping, dtype=np.int32)
+            keep.append(mapping)
         self._keep.append((s, keep))
         mid = C.c_int()
-        _chk(lib().swe_srv_add_mesh(self._h, C.byref(s), C.byref(mid)), "swe_srv_add_mesh")
+        _chk(lib().swe_srv_add_mesh_ex(self._h, C.byref(s),
+                                       mapping.ctypes.data_as(_ip) if mapping is not None
+                                       else None, int(prob.n_nodes), C.byref(mid)),
+             "swe_srv_add_mesh_ex")
         self.problems.append(prob)
         return mid.value
 
@@ -124,8 +137,9 @@ class Server:
         pin = (_dp * len(inits))(*[a.ctypes.data_as(_dp) for a in inits])
         pout = (_dp * len(inits))(*[a.ctypes.data_as(_dp) for a in req.results])
         rid = C.c_int()
-        _chk(lib().swe_srv_submit(self._h, req.mesh, len(inits), pin, pout, int(req.nsteps),
-                                  int(req.tag), C.byref(rid)), "swe_srv_submit")
+        path = req.result_path.encode() if req.result_path else None
+        _chk(lib().swe_srv_submit_ex(self._h, req.mesh, len(inits), pin, pout, int(req.nsteps),
+                                     int(req.tag), path, C.byref(rid)), "swe_srv_submit_ex")
         self._live[req.tag] = (req, inits, pin, pout)
         return rid.value
 

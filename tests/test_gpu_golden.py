@@ -81,3 +81,36 @@ def test_driver_history_and_known_answer(pkg, tmp_path, mode):
     assert np.array_equal(states[0], g["H10"])
     raw = pkg.mesh.read_bin(CASE)
     assert np.array_equal(states[-1], raw.H_final)
+
+
+@pytest.mark.parametrize("mode", ["substep", "steps"])
+def test_driver_restart_continues_bitwise(pkg, tmp_path, mode):
+    """Restart from a history snapshot (SWE_RESTART_FILE/STEP): the continued
+    run's snapshots equal the uninterrupted run's bit for bit, and its final
+    state still passes the known-answer check."""
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "rbf-fd-swe_amd", "swe_run")
+    base = dict(os.environ, SWE_INPUT_FILE=CASE, SWE_USE_NETCDF="0", SWE_NUM_COMPUTE_UNITS="1",
+                SWE_NSTEPS="100", SWE_NATTEMPTS="1", SWE_USE_RCM="1", SWE_USE_OCL="0",
+                SWE_LAYOUT="cfdl", SWE_HIP_MODE=mode, SWE_HIP_EXACT="1", SWE_HIST_INT="10")
+    full, cont = tmp_path / "full.bin", tmp_path / "cont.bin"
+    print(f"case: swe_run restart {mode}", flush=True)
+    r = subprocess.run([exe], env=dict(base, SWE_OUTPUT_FILE=str(full)), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    r = subprocess.run([exe], env=dict(base, SWE_OUTPUT_FILE=str(cont),
+                                       SWE_RESTART_FILE=str(full), SWE_RESTART_STEP="50",
+                                       SWE_NATTEMPTS="2"),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Restart from step 50" in r.stdout
+    assert r.stdout.count("VERIFICATION SUCCEEDED") == 2
+    s_full, _, h_full = pkg.mesh.read_history(str(full))
+    s_cont, _, h_cont = pkg.mesh.read_history(str(cont))
+    assert s_cont == list(range(60, 101, 10))
+    assert np.array_equal(h_cont, h_full[5:])
+    bad = subprocess.run([exe], env=dict(base, SWE_RESTART_FILE=str(full),
+                                         SWE_RESTART_STEP="55"),
+                         capture_output=True, text=True, timeout=300)
+    assert bad.returncode == 2 and "restart snapshot" in bad.stdout

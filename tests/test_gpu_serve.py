@@ -101,3 +101,28 @@ def test_tiled_batches_equal_solo(pkg):
     finally:
         for c in ctxs:
             c.close()
+
+
+def test_result_files_hold_each_request(pkg, tmp_path):
+    """Per-request result files (swe_srv_submit_ex): the history format of
+    swe_run, one record per member in the file node order, equal to the
+    results returned in memory."""
+    probs = _probs(pkg)
+    reqs = _mix(pkg, probs, 6, seed=21, max_members=3)
+    for r in reqs:
+        r.result_path = str(tmp_path / f"req{r.tag}.bin")
+    print("case: serving result files", flush=True)
+    with pkg.serve.Server(slots=4, member_block=2, chunk_steps=5) as srv:
+        for p in probs:
+            srv.add_mesh(p)
+        stats = srv.run(reqs)
+    assert stats["requests_done"] == len(reqs)
+    for r in reqs:
+        prob = probs[r.mesh]
+        steps, times, states = pkg.mesh.read_history(r.result_path)
+        assert steps == [r.nsteps] * len(r.inits)
+        for m, H in enumerate(r.results):
+            Hn = prob.to_node_major(H)[: prob.n_nodes]
+            nat = np.empty_like(Hn)
+            nat[prob.mapping] = Hn
+            assert np.array_equal(states[m], nat), (r.tag, m)
