@@ -1,25 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark of the RBF-FD shallow-water RK4 hot path on one MI355X.
 
-Workload (default): the N=40962 icosahedral mesh, 31-point stencils, FP64,
-one state, RK4 with the fused graph-captured stage kernels.  One "step" is
-one RK4 timestep = 4 eval_RHS passes over every node.  Inputs are generated
-(seeded) in the reference's binary format, prepared by the C host (RCM,
-layout) and resident in HBM before the timed region.
-
-Prints one JSON line:
-  value      eval_RHS throughput, GNode-RHS/s (N * 4 * steps / time)
-  roofline   the dominant kernel (the fused stage kernel) against HBM peak:
-             algorithmic bytes per launch / average launch duration, the
-             duration taken from HIP events recorded on the context's own
-             stream around the timed region
-  cpu_baseline  the reference's own CPU path (oracle/_ref/swe_ref_cfdl.exe,
-             built from the reference sources with -O3 -fopenmp
-             -DRHS_INNER_SIMD) on the same input, timed by its own
-             profiling report, on a bounded sample of steps
+Prints one JSON line.  Headline (the contract's value/roofline/cpu_baseline):
+the N=40962 icosahedral mesh, 31-point stencils, FP64, one state, RK4 with
+the fused graph-captured stage kernels; one "step" is one RK4 timestep = 4
+eval_RHS passes over every node; inputs are generated (seeded) in the
+reference's binary format, prepared by the C host (RCM, layout) and resident
+in HBM before the timed region.
+  value         eval_RHS throughput, GNode-RHS/s (N * 4 * steps / time)
+  roofline      the stage kernel against HBM peak: algorithmic bytes per
+                launch / average launch duration, timed with HIP events on
+                the context's own stream around the timed region
+  cpu_baseline  the reference's own main() (oracle/_ref/swe_ref_cfdl.exe,
+                built from the reference sources with -O3 -fopenmp
+                -DRHS_INNER_SIMD) on the same input, timed by its own
+                profiling report on a bounded step sample; cpu_baseline_sfdl
+                the same for the sfdl layout
+Further legs (each its own key, none of them the headline):
+  c3         N=655362, 200 steps, the single-state kernel the autotuner picks
+  ensemble   N=163842 x 64 members, 500 steps, the ensemble configuration the
+             autotuner picks; member-steps/s, ratio to the single-member path
+             on the same mesh, and its HBM and FP64 roofline fractions
+  serving    2000-request Poisson mix (N in {2562, 10242, 40962, 163842},
+             1-32 members, 50-500 steps), HBM cap 200 GB: p50/p99 latency,
+             aggregate member-steps/s, makespan; every result checked finite
+             and a sample checked bitwise against solo runs
+  fp32       the headline workload in the reference's FP32 model type
 
 Flags: --gpus N (only 1 is supported: every GPU box here has one GPU),
---steps K, --warmup W, --nodes N, --members B, --exact, --no-cpu.
+--steps K, --warmup W, --nodes N, --members B, --exact, --no-cpu,
+--no-ensemble, --no-serve, --no-c3, --no-fp32.
 """
 import argparse
 import json
@@ -79,14 +89,18 @@ def parse_args():
                     help="only run 4 device copies of 1 GiB (HBM counter calibration) and exit")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble leg (N=163842, 64 members, 500 steps)")
-    ap.add_argument("--serve", action="store_true",
-                    help="add the serving leg (Poisson mix of forecast requests)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving leg (Poisson mix of forecast requests)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the N=655362 single-state leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-rate", type=float, default=25.0)
+    ap.add_argument("--serve-rate", type=float, default=9.0,
+                    help="Poisson arrival rate, requests/s (9/s: about the rate one MI355X "
+                         "completes this mix at)")
     ap.add_argument("--serve-kernel", default="tiles", choices=["tiles", "pairs"],
                     help="batch kernel of the serving leg: member-lane LDS tiles (member "
                          "block 32) or member pairs (member block 64)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     return ap.parse_args()
 
 
@@ -99,10 +113,10 @@ def cpu_cores() -> int:
     return min(n, int(env)) if env and env.isdigit() else n
 
 
-def run_reference_cpu(path: str, n_nodes: int, target_s: float):
+def run_reference_cpu(path: str, n_nodes: int, target_s: float, layout: str = "cfdl"):
     """Times the reference build's own main() (Eval_Rhs seconds/timestep
     from its profiling report, attempts >= 2) on a bounded step sample."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "swe_ref_cfdl.exe")
+    exe = os.path.join(ROOT, "oracle", "_ref", f"swe_ref_{layout}.exe")
     if not os.path.exists(exe):
         return None
     cores = cpu_cores()
@@ -126,7 +140,7 @@ def run_reference_cpu(path: str, n_nodes: int, target_s: float):
         return None
     return {"value": n_nodes * 4 / per_step / 1e9, "unit": "GNode-RHS/s", "cores": cores,
             "kind": "reference",
-            "sample": (f"reference main() (-O3 -fopenmp -DRHS_INNER_SIMD, FP64, cfdl, RCM) on the "
+            "sample": (f"reference main() (-O3 -fopenmp -DRHS_INNER_SIMD, FP64, {layout}, RCM) on the "
                        f"same N={n_nodes} input: {attempts} attempts x {steps} steps, Eval_Rhs "
                        f"s/step averaged over attempts >= 2 = {per_step:.4e} s "
                        f"({wall:.1f} s wall)"),
@@ -147,65 +161,108 @@ def dominant_kernel(args):
         else "k_stage_members"
 
 
+FP64_PEAK_TFLOPS = 78.6
+FLOP_PER_NODE_RHS = 32 * NNBR + 66   # 16 stencil sums of 31 terms + the pointwise RHS
+
+
+def timed_steps(torch, ctx, k, warm=2):
+    """ms per fused RK step over k steps, HIP events on the context's stream."""
+    st = torch.cuda.ExternalStream(ctx.stream())
+    ctx.rk_steps(warm)
+    ctx.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    ctx.rk_steps(k)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / k
+
+
+def describe(opts):
+    if opts.get("members", 1) == 1:
+        if opts["kernel"] == "tile":
+            return (f"k_stage_ptile (pair mapping over LDS tiles of {opts['tile_nodes']} nodes, "
+                    f"row staging {opts['prefetch']})")
+        return "k_stage_pair"
+    if opts["tiled"] and opts["member_block"] == 32:
+        return (f"k_stage_tspec (member-lane LDS tiles, {opts['tile_nodes'] or 8} nodes x "
+                f"{opts['tile_rows'] or 70} rows, loader and compute waves, direct-to-LDS loads)")
+    if opts["tiled"]:
+        return f"k_stage_tiled (member block {opts['member_block']})"
+    if opts["member_block"] == 64:
+        return "k_stage_mpair (member pairs, 16-byte gathers)"
+    return f"k_stage_members (member block {opts['member_block']})"
+
+
+def c3_leg(pkg, torch, steps=200):
+    """The largest mesh (N=655362), single state, 200 steps, on the kernel
+    the autotuner picks for it (Hilbert-ordered device copy)."""
+    import numpy as np
+    path = pkg.mesh.case_path(655362, seed=1)
+    prob = pkg.host.Problem(path, use_rcm=True, layout="cfdl")
+    with pkg.runtime.HipContext(prob, order="hilbert") as base:
+        opts, _ = base.tune(1, steps=3)
+        with pkg.runtime.HipContext(prob, share=base, **opts) as ctx:
+            ctx.set_state(prob.H_init)
+            ms = timed_steps(torch, ctx, steps)
+            finite = bool(np.isfinite(ctx.get_state()).all())
+    n = prob.n_nodes
+    launch_us = ms * 1e3 / 4
+    achieved = stage_bytes(prob.padded, 1) / (launch_us * 1e-6) / 1e9
+    return {"workload": f"N={n}, 1 member, {steps} RK4 steps, FP64, fused stages + HIP graphs",
+            "kernel": describe(opts), "options": opts, "ms_per_step": ms,
+            "gnode_rhs_per_s": n * 4 / (ms * 1e-3) / 1e9,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "bytes_per_launch": stage_bytes(prob.padded, 1),
+                         "avg_launch_us": launch_us},
+            "state_finite": finite}
+
+
 def ensemble_leg(pkg, torch, steps=500):
     """64-member ensemble at N=163842 against the single-state path on the
-    same mesh.  The ensemble kernel is picked by timing the candidates (member
-    pairs at two pipeline depths, LDS tiles) on this mesh."""
+    same mesh; the ensemble configuration is the autotuner's pick."""
     import numpy as np
     prob = pkg.host.Problem(pkg.mesh.case_path(163842, seed=1), use_rcm=True, layout="cfdl")
     prob.path = pkg.mesh.case_path(163842, seed=1)
     inits = pkg.serve.member_pool(pkg.mesh, prob, 64, seed=4)
-
-    def timed(ctx, k):
-        st = torch.cuda.ExternalStream(ctx.stream())
-        ctx.rk_steps(2)
-        ctx.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        ctx.rk_steps(k)
-        e1.record(st)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / k
-    single = pkg.runtime.HipContext(prob)
-    single.set_state(inits[0])
-    ms1 = timed(single, steps)
-    base = pkg.runtime.HipContext(prob, order="hilbert")
-    candidates = [("member pairs (16-byte gathers), pipeline depth 3",
-                   dict(member_block=64, prefetch=0)),
-                  ("LDS tiles with member lanes, member block 32, loader and compute waves "
-                   "over two LDS images, persistent",
-                   dict(member_block=32, tiled=True))]
-    best = None
-    for name, kw in candidates:
-        ctx = pkg.runtime.HipContext(prob, members=64, share=base, **kw)
-        for m in range(64):
-            ctx.set_state(inits[m], m)
-        ms = timed(ctx, 20)
-        if best is None or ms < best[2]:
-            best = (name, kw, ms)
-        ctx.close()
-    ens = pkg.runtime.HipContext(prob, members=64, share=base, **best[1])
-    for m in range(64):
-        ens.set_state(inits[m], m)
-    ms = timed(ens, steps)
-    finite = bool(np.isfinite(ens.get_state(63)).all())
-    ens.close()
-    base.close()
-    single.close()
-    n = prob.n_nodes
+    with pkg.runtime.HipContext(prob) as single:
+        single.set_state(inits[0])
+        ms1 = timed_steps(torch, single, steps)
+    with pkg.runtime.HipContext(prob, order="hilbert") as base:
+        opts, _ = base.tune(64, steps=3)
+        with pkg.runtime.HipContext(prob, share=base, **opts) as ens:
+            for m in range(64):
+                ens.set_state(inits[m], m)
+            ms = timed_steps(torch, ens, steps)
+            finite = all(bool(np.isfinite(ens.get_state(m)).all()) for m in range(64))
+    n, N = prob.n_nodes, prob.padded
+    launch_us = ms * 1e3 / 4
+    bytes_launch = stage_bytes(N, 64)
+    flops_launch = float(N) * 64 * FLOP_PER_NODE_RHS
+    achieved = bytes_launch / (launch_us * 1e-6) / 1e9
+    tflops = flops_launch / (launch_us * 1e-6) / 1e12
     return {"workload": f"N={n}, 64 members (seeded 1e-3 perturbations), {steps} RK4 steps, "
-                        f"{best[0]}, Hilbert-ordered device copy",
+                        "Hilbert-ordered device copy, fused stages + HIP graphs",
+            "kernel": describe(opts), "options": opts,
             "member_steps_per_s": 64 / (ms * 1e-3), "ms_per_step": ms,
             "gnode_rhs_per_s": n * 4 * 64 / (ms * 1e-3) / 1e9,
-            "fp64_tflops": n * 64 * 4 * 1058 / (ms * 1e-3) / 1e12,
             "single_member_steps_per_s": 1 / (ms1 * 1e-3),
-            "ratio_vs_single": (64 / ms) / (1 / ms1), "state_finite": finite}
+            "ratio_vs_single": (64 / ms) / (1 / ms1),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "bytes_per_launch": bytes_launch, "avg_launch_us": launch_us,
+                         "fp64_tflops": tflops, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                         "fp64_frac": tflops / FP64_PEAK_TFLOPS},
+            "state_finite": finite}
 
 
-def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026):
+def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026, sample_every=250):
     """Continuous-batching mix: Poisson arrivals, N in {2562, 10242, 40962,
-    163842}, 1-32 members, 50-500 steps, HBM cap 200 GB."""
+    163842}, 1-32 members, 50-500 steps, HBM cap 200 GB.  Every returned
+    state is checked finite; every sample_every-th request is kept and
+    re-run alone afterwards, which must give the same bits."""
     import numpy as np
     probs, pools = [], []
     for n in (2562, 10242, 40962, 163842):
@@ -215,13 +272,16 @@ def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026):
         probs.append(p)
         pools.append(pkg.serve.member_pool(pkg.mesh, p, 32, seed=n))
     reqs = pkg.serve.poisson_mix(pools, n_requests, rate, seed)
-    lat, que, done = [], [], [0]
+    lat, que, done, kept = [], [], [0], []
 
     def on_done(r):
         lat.append(r.latency_s)
         que.append(r.queue_s)
-        if not all(np.isfinite(x[:8]).all() for x in r.results):
-            raise RuntimeError(f"request {r.tag} returned non-finite state")
+        for x in r.results:
+            if not np.isfinite(x).all():
+                raise RuntimeError(f"request {r.tag} returned a non-finite state")
+        if r.tag % sample_every == 0:
+            kept.append((r.mesh, r.nsteps, list(r.inits), list(r.results)))
         r.results = None
         r.inits = None
         done[0] += 1
@@ -235,6 +295,19 @@ def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026):
         for p in probs:
             srv.add_mesh(p)
         stats = srv.run(reqs, on_done=on_done)
+    checked = 0
+    ctxs = [pkg.runtime.HipContext(p) for p in probs]
+    try:
+        for mesh, nsteps, inits, results in kept:
+            for H0, got in zip(inits, results):
+                ctxs[mesh].set_state(H0)
+                ctxs[mesh].rk_steps(nsteps)
+                if not np.array_equal(ctxs[mesh].get_state(), got):
+                    raise RuntimeError("a served result differs from its solo run")
+                checked += 1
+    finally:
+        for c in ctxs:
+            c.close()
     lat = np.array(lat)
     return {"workload": f"{n_requests} requests, Poisson {rate}/s, N in {{2562,10242,40962,163842}}, "
                         "1-32 members, 50-500 steps, HBM cap 200 GB, batches of 64 slots on "
@@ -245,7 +318,35 @@ def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026):
             "member_steps_per_s": work / stats["makespan_s"],
             "gnode_rhs_per_s": node_rhs / stats["makespan_s"] / 1e9,
             "makespan_s": stats["makespan_s"], "hbm_peak_gb": stats["hbm_peak"] / 2**30,
-            "batches": stats["batches"], "chunks": stats["chunks_launched"]}
+            "batches": stats["batches"], "chunks": stats["chunks_launched"],
+            "evictions": stats["evictions"],
+            "results_checked": "every state finite; "
+                               f"{checked} members of {len(kept)} sampled requests bitwise "
+                               "equal to their solo runs"}
+
+
+def fp32_leg(pkg, torch, nodes, steps, warmup):
+    """The headline workload in the reference's FP32 model type (its shipped
+    precision, RUN_32BIT)."""
+    import numpy as np
+    prob = pkg.host.Problem(pkg.mesh.case_path(nodes, seed=1), use_rcm=True, layout="cfdl",
+                            fp32=True)
+    with pkg.runtime.HipContext(prob, fp32=True) as ctx:
+        ctx.set_state(prob.H_init)
+        ms = timed_steps(torch, ctx, steps, warm=warmup)
+        finite = bool(np.isfinite(ctx.get_state()).all())
+    launch_us = ms * 1e3 / 4
+    achieved = stage_bytes(prob.padded, 1, fp32=True) / (launch_us * 1e-6) / 1e9
+    return {"workload": f"N={prob.n_nodes}, 1 member, {steps} RK4 steps, FP32 "
+                        "(the reference's RUN_32BIT type)",
+            "kernel": "k_stage_lane32", "dtype": "f32",
+            "value": prob.n_nodes * 4 / (ms * 1e-3) / 1e9, "unit": "GNode-RHS/s",
+            "ms_per_step": ms,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "bytes_per_launch": stage_bytes(prob.padded, 1, fp32=True),
+                         "avg_launch_us": launch_us},
+            "state_finite": finite}
 
 
 def main():
@@ -307,22 +408,39 @@ def main():
     launches = 4 * args.steps
     launch_s = gpu_ms * 1e-3 / launches
     achieved = stage_bytes(N, args.members, args.fp32) / launch_s / 1e9
-    traffic = None
+    # HBM bytes per launch are counted by rocprofv3 (FETCH_SIZE, WRITE_SIZE)
+    # outside this process; the stored count is reported with its source and
+    # only when it was taken for the kernel this run selects
+    traffic, traffic_source = None, None
     tpath = os.path.join(ROOT, "profiles", f"traffic_n{args.nodes}_b{args.members}.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and not args.fp32:
         with open(tpath) as fp:
-            traffic = json.load(fp).get("hbm_bytes_per_launch")
+            rec = json.load(fp)
+        if rec.get("kernel") == dominant_kernel(args):
+            traffic = rec.get("hbm_bytes_per_launch")
+            traffic_source = (f"stored rocprofv3 --pmc count of {rec.get('kernel')} "
+                              f"({os.path.relpath(tpath, ROOT)}), not measured in this run")
     node_rhs = float(prob.n_nodes) * 4 * args.steps * args.members
     value = node_rhs / (gpu_ms * 1e-3) / 1e9
     ctx.close()
-    cpu = None
+    cpu = cpu_sfdl = None
     if not args.no_cpu:
-        cpu = run_reference_cpu(path, prob.n_nodes, args.cpu_seconds)
+        cpu = run_reference_cpu(path, prob.n_nodes, args.cpu_seconds, "cfdl")
+        cpu_sfdl = run_reference_cpu(path, prob.n_nodes, args.cpu_seconds, "sfdl")
+        print("bench: CPU baselines done", file=sys.stderr, flush=True)
+    fp32 = None
+    if not args.no_fp32 and not args.fp32 and args.members == 1:
+        fp32 = fp32_leg(pkg, torch, args.nodes, args.steps, args.warmup)
+    c3 = None
+    if not args.no_c3:
+        c3 = c3_leg(pkg, torch)
+        print("bench: C3 leg done", file=sys.stderr, flush=True)
     ensemble = None
     if not args.no_ensemble:
         ensemble = ensemble_leg(pkg, torch)
+        print("bench: ensemble leg done", file=sys.stderr, flush=True)
     serving = None
-    if args.serve:
+    if not args.no_serve:
         serving = serve_leg(pkg, args.serve_requests, args.serve_rate, args.serve_kernel)
     out = {
         "metric": "eval_RHS GNodes/s + HBM GB/s; ensemble steps/s and p50/p99 latency",
@@ -335,10 +453,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if args.fp32 else "f64",
         "data": "synthetic (seeded icosahedral mesh + Gaussian RBF-FD weights, TC5-like state)",
         "config": {"workload": f"RBF-FD SWE RK4, N={prob.n_nodes} nodes, stencil 31, "
-                               f"{args.members} member(s), fused stage kernels + HIP graphs",
+                               f"{args.members} member(s), fused stage kernels + HIP graphs "
+                               "(the 40962-node, 1000-step single-member case timed against "
+                               "the reference CPU path)",
                    "nodes": prob.n_nodes, "members": args.members, "layout": args.layout,
                    "member_block": args.member_block, "kernel": args.kernel,
                    "prefetch": args.prefetch, "order": args.order, "tiled": args.tiled,
@@ -346,12 +466,16 @@ def main():
                    "exact": bool(args.exact), "dt": prob.dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_source,
                      "kernel": dominant_kernel(args),
                      "bytes_per_launch": stage_bytes(N, args.members, args.fp32),
                      "avg_launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,
+        "cpu_baseline_sfdl": cpu_sfdl,
+        "c3": c3,
         "ensemble": ensemble,
         "serving": serving,
+        "fp32": fp32,
         "member_steps_per_s": args.members * args.steps / (gpu_ms * 1e-3),
         "wall_s_timed_region": wall,
         "state_finite": finite,

@@ -186,6 +186,20 @@ int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, in
 int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int steps,
                           int *best_mb, int *best_tiled, double *best_ms_per_step);
 
+/* Autotuner over the whole configuration space: times `steps` fused RK
+ * steps of every candidate on base's static data and node order and writes
+ * the fastest, ready for swe_hip_create_ex (share = base), to *best:
+ *   members == 1: the PAIR kernel and the TILE kernel with tiles of 64, 128
+ *     and 256 nodes and each row-staging schedule (prefetch 0: through
+ *     registers, 4: direct-to-LDS, 8: direct-to-LDS with the first weights
+ *     loaded before the tile barrier);
+ *   members > 1: every member block, untiled and LDS-tiled, and for member
+ *     block 32 the member-lane tile shapes (tile_nodes 8 or 6 x tile_rows 70
+ *     or 56).
+ * FP64 contexts.  No counterpart in the reference. */
+int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
+                    swe_hip_options *best, double *best_ms_per_step);
+
 /* Per-attempt state upload (main.c:297 + 304-315): H in the host state
  * layout (SVM_LIN_ID, patch_size*4 values) into member slot `member`. */
 int swe_hip_set_state(swe_hip_ctx *ctx, int member, const double *H);

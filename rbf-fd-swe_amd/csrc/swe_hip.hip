@@ -667,13 +667,14 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
 
 namespace {
 
-// Milliseconds per fused RK step of one candidate (member block, tiled) on the
+// Milliseconds per fused RK step of one candidate configuration on the
 // static data of `base`: one warm-up step (graph capture), then `steps`
 // timed with HIP events on the candidate's stream.
-int time_candidate(swe_hip_ctx *base, int members, unsigned flags, int mb, int tiled, int steps,
-                   double *per_step)
+int time_options(swe_hip_ctx *base, const swe_hip_options &opt, int steps, double *per_step)
 {
-    swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, tiled, 0, 0, 0};
+    swe_hip_options o = opt;
+    o.share = base;
+    o.device = base->dev;
     swe_hip_ctx *c = nullptr;
     int rc = swe_hip_create_ex(nullptr, &o, &c);
     if (rc)
@@ -717,7 +718,8 @@ int tune_candidates(swe_hip_ctx *base, int members, unsigned flags, int steps, b
             if (tiled && !((mb >= 4 && mb <= 16) || (mb == 32 && base->nnbr == 31)))
                 continue;
             double per = 0.0;
-            const int rc = time_candidate(base, members, flags, mb, tiled, steps, &per);
+            const swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, tiled, 0, 0, 0};
+            const int rc = time_options(base, o, steps, &per);
             if (rc)
                 return rc;
             if (per < best) {
@@ -983,6 +985,55 @@ int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int st
     if (!base || members < 2 || steps < 1 || !best_mb || !best_tiled)
         return fail(SWE_HIP_EINVAL, "tune_ensemble: bad arguments");
     return tune_candidates(base, members, flags, steps, true, best_mb, best_tiled, best_ms);
+}
+
+int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
+                    swe_hip_options *best, double *best_ms)
+{
+    if (!base || members < 1 || steps < 1 || !best || (flags & SWE_HIP_FP32) || base->fp32)
+        return fail(SWE_HIP_EINVAL, "tune_ex: bad arguments (FP64 contexts)");
+    std::vector<swe_hip_options> cand;
+    auto add = [&](int mb, int kernel, int prefetch, int tiled, int tn, int tr) {
+        cand.push_back(swe_hip_options{members, flags, base->dev, mb, base, kernel, prefetch, 0,
+                                       tiled, 0, tn, tr});
+    };
+    if (members == 1) {
+        add(0, SWE_HIP_KERNEL_PAIR, 0, 0, 0, 0);
+        if (base->nnbr == 31)
+            for (int tn : {64, 128, 256})
+                for (int pf : {0, 4, 8})
+                    add(0, SWE_HIP_KERNEL_TILE, pf, 0, tn, 0);
+    } else {
+        for (int mb = 64; mb >= 2; mb >>= 1) {
+            if (members % mb)
+                continue;
+            add(mb, 0, 0, 0, 0, 0);
+            if (mb >= 4 && mb <= 16)
+                add(mb, 0, 0, 1, 0, 0);
+            if (mb == 32 && base->nnbr == 31)   // member-lane tiles: nodes x rows per tile
+                for (int tn : {8, 6})
+                    for (int tr : {70, 56})
+                        add(mb, 0, 0, 1, tn, tr);
+        }
+    }
+    double best_t = 1e300;
+    int found = -1;
+    for (size_t k = 0; k < cand.size(); ++k) {
+        double per = 0.0;
+        const int rc = time_options(base, cand[k], steps, &per);
+        if (rc)
+            return rc;
+        if (per < best_t) {
+            best_t = per;
+            found = (int)k;
+        }
+    }
+    if (found < 0)
+        return fail(SWE_HIP_EINVAL, "tune_ex: members has no admissible configuration");
+    *best = cand[found];
+    if (best_ms)
+        *best_ms = best_t;
+    return 0;
 }
 
 int swe_hip_info(const swe_hip_ctx *c, int *n_nodes, int *members, int *nnbr, unsigned *flags)

@@ -31,6 +31,7 @@ EXPORTS = (
     "swe_hip_tune_member_block",
     "swe_hip_tune_ensemble", "swe_hip_rk_steps_noalloc",
     "swe_hip_create32", "swe_hip_set_state32", "swe_hip_get_state32", "swe_hip_eval_rhs32",
+    "swe_hip_tune_ex",
 )
 
 
@@ -89,6 +90,8 @@ def lib() -> C.CDLL:
                                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.swe_hip_tune_ensemble.argtypes = [vp, C.c_int, C.c_uint, C.c_int, C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.swe_hip_tune_ex.argtypes = [vp, C.c_int, C.c_uint, C.c_int, C.POINTER(SweHipOptions),
+                                      C.POINTER(C.c_double)]
         L.swe_hip_query.argtypes = [vp]
         L.swe_hip_set_active.argtypes = [vp, C.c_int]
         L.swe_hip_move_member.argtypes = [vp, C.c_int, C.c_int]
@@ -328,6 +331,19 @@ class HipContext:
                                          C.byref(mb), C.byref(tiled), C.byref(ms)),
              "swe_hip_tune_ensemble")
         return mb.value, bool(tiled.value), ms.value
+
+    def tune(self, members: int = 1, steps: int = 5, exact: bool = False):
+        """(options dict for HipContext(..., share=self, **opts), ms per step):
+        the fastest configuration swe_hip_tune_ex finds on this mesh."""
+        o = SweHipOptions()
+        ms = C.c_double()
+        _chk(lib().swe_hip_tune_ex(self.handle, int(members), SWE_HIP_EXACT if exact else 0,
+                                   int(steps), C.byref(o), C.byref(ms)), "swe_hip_tune_ex")
+        kname = {v: k for k, v in KERNELS.items()}[o.kernel]
+        opts = dict(members=o.members, member_block=o.member_block, kernel=kname,
+                    prefetch=o.prefetch, tiled=o.tiled, tile_nodes=o.tile_nodes,
+                    tile_rows=o.tile_rows)
+        return opts, ms.value
 
     def set_active(self, active: int) -> None:
         _chk(lib().swe_hip_set_active(self.handle, int(active)), "swe_hip_set_active")
