@@ -148,7 +148,10 @@ typedef struct swe_hip_options {
     int kernel;                 /* SWE_HIP_KERNEL_* (members == 1 only)      */
     int prefetch;               /* stencil-loop schedule: 0 (compiler's),
                                    4 or 8 (ring of slots; member-pair kernel:
-                                   ring depth 2 or 4, default 3)            */
+                                   ring depth 2 or 4, default 3); member-lane
+                                   tiles without SWE_HIP_EXACT: 8 takes the
+                                   stencil sums on the FP64 matrix cores
+                                   (v_mfma_f64_4x4x4f64)                     */
     int node_order;             /* SWE_HIP_ORDER_* of the device copy (ignored
                                    with share: the mesh's order is used)    */
     int tiled;                  /* 1: LDS-staged ensemble kernel (members

@@ -23,8 +23,12 @@ void launch_tspec(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
     const long items = (long)tl.ntiles * tl.nmb;
     const long wgs = c->ncu;
     st.nblk = (int)(((items < wgs ? items : wgs) + 7) / 8 * 8);
-    const size_t lds = sizeof(double) * 2 * (size_t)kTmemItemDoubles;
-    k_stage_tspec<STAGE, EX, 31><<<dim3(st.nblk), dim3(512), lds, c->stream>>>(m, tl, st);
+    const size_t lds = sizeof(double) * (size_t)kTmemLdsDoubles;
+    if (!EX && c->prefetch == 8)   // stencil sums on the FP64 matrix cores
+        k_stage_tspec<STAGE, false, 31, true><<<dim3(st.nblk), dim3(512), lds, c->stream>>>(
+            m, tl, st);
+    else
+        k_stage_tspec<STAGE, EX, 31><<<dim3(st.nblk), dim3(512), lds, c->stream>>>(m, tl, st);
 }
 
 }  // namespace

@@ -893,7 +893,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
                                           "a 31-point stencil");
             else if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                            device) != hipSuccess ||
-                     (size_t)lds_max < sizeof(double) * 2 * (size_t)kTmemItemDoubles)
+                     (size_t)lds_max < sizeof(double) * (size_t)kTmemLdsDoubles)
                 rc = fail(SWE_HIP_EDEVICE, "the member-lane tiled kernel needs 160 KB of LDS "
                                            "per workgroup");
             else
@@ -1012,7 +1012,7 @@ int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
                 add(mb, 0, 0, 1, 0, 0);
             if (mb == 32 && base->nnbr == 31)   // member-lane tiles: nodes x rows per tile
                 for (int tn : {8, 6})
-                    for (int tr : {70, 56})
+                    for (int tr : {kTmemRows, 56})
                         add(mb, 0, 0, 1, tn, tr);
         }
     }

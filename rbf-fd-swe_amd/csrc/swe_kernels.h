@@ -1203,6 +1203,9 @@ __device__ __forceinline__ void tmem_dma_issue(double *img, const DevTiles &tl,
     }
 }
 
+// LDS of one workgroup: two item images.
+constexpr int kTmemLdsDoubles = 2 * kTmemItemDoubles;
+
 // The next-item header (geometry slots kTmemHdrSlot, +1 of wave slice wid's
 // first row), written after the slice's direct-to-LDS loads completed.
 __device__ __forceinline__ void tmem_dma_header(double *img, const TmemItem &next, int wid,
@@ -1267,9 +1270,97 @@ __device__ __forceinline__ void stage_update_pre_pair(const DevStage &st, long e
     }
 }
 
+// x's upper 32 lanes <-> y's lower 32 lanes (v_permlane32_swap, per dword).
+__device__ __forceinline__ void swap_rows32(double &x, double &y)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x),
+                                                     (unsigned)__double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x),
+                                                     (unsigned)__double2hiint(y), false, false);
+    x = __hiloint2double((int)hi[0], (int)lo[0]);
+    y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// x's odd 16-lane rows <-> y's even rows (v_permlane16_swap, per dword).
+__device__ __forceinline__ void swap_rows16(double &x, double &y)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x),
+                                                     (unsigned)__double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x),
+                                                     (unsigned)__double2hiint(y), false, false);
+    x = __hiloint2double((int)hi[0], (int)lo[0]);
+    y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// Stencil sums of wave slice wid's two nodes on the FP64 matrix cores
+// (fast mode).  Per node and group of 4 slots, v_mfma_f64_4x4x4f64 (4
+// blocks b = (lane >> 2) & 3; A[i][k] in lane 16 k + 4 b + i, B[k][j] in
+// lane 16 k + 4 b + j, D[i][j] in lane 16 i + 4 b + j) multiplies the 4 x 4
+// weights (operator i, slot k) by 16 (variable, member) columns of the
+// neighbour states: instruction g takes variable g >> 1, members 16 (g & 1)
+// + 4 b + j.  8 slot groups x 8 instructions per node; slot 31 has zero
+// weights (its position, the node's own row, is any valid row).  Lane row r
+// then holds operator r's sums for 2 nodes x 2 member halves x 4 variables;
+// a 4 x 4 transpose over the lane rows (row pairs 32 apart, then 16 apart)
+// leaves lane (node half, member ml) with the 16 sums of its member.  The
+// instruction sums 4 products per slot group, not the sequential sums of
+// the vector path.
+template <int NNBR>
+__device__ __forceinline__ Sums tmem_sums_mfma(const double *img, int wid, int lane)
+{
+    static_assert(NNBR <= 31, "8 groups of 4 slots, slot 31 weighted zero");
+    const double *wl = img + wid * 256;                         // [node][slot][op]
+    const double *gl = img + kTmemWtsDoubles + wid * 64;        // [node][32]
+    const double *rows = img + kTmemWtsDoubles + kTmemGeoDoubles;
+    const int k = lane >> 4, i = lane & 3, c = lane & 15;
+    double acc[2][8];
+#pragma unroll
+    for (int n = 0; n < 2; ++n) {
+        const unsigned short *pos16 =
+            reinterpret_cast<const unsigned short *>(gl + n * 32 + kTmemPosSlot);
+#pragma unroll
+        for (int g = 0; g < 8; ++g)
+            acc[n][g] = 0.0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int slot = 4 * s + k;
+            const double a = (slot < NNBR) ? wl[n * 128 + slot * 4 + i] : 0.0;
+            unsigned p = pos16[slot];
+            p = p == 0xffffu ? 0u : p;   // no own row (padding): any row, zero weight
+            const double *rb = rows + p * 128 + c;
+            double b[8];
+#pragma unroll
+            for (int g = 0; g < 8; ++g)
+                b[g] = rb[(g >> 1) * 32 + 16 * (g & 1)];
+#pragma unroll
+            for (int g = 0; g < 8; ++g)
+                acc[n][g] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b[g], acc[n][g], 0, 0, 0);
+        }
+    }
+    // per variable v: M[row r][q] = acc[q >> 1][2 v + (q & 1)] -> lane row q
+    // gets M[0..3][q]
+    double T[4][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        double x0 = acc[0][2 * v], x1 = acc[0][2 * v + 1];
+        double x2 = acc[1][2 * v], x3 = acc[1][2 * v + 1];
+        swap_rows32(x0, x2);
+        swap_rows32(x1, x3);
+        swap_rows16(x0, x1);
+        swap_rows16(x2, x3);
+        T[v][0] = x0; T[v][1] = x1; T[v][2] = x2; T[v][3] = x3;
+    }
+    Sums S;
+    S.dudx = T[0][0]; S.dudy = T[0][1]; S.dudz = T[0][2]; S.lu = T[0][3];
+    S.dvdx = T[1][0]; S.dvdy = T[1][1]; S.dvdz = T[1][2]; S.lv = T[1][3];
+    S.dwdx = T[2][0]; S.dwdy = T[2][1]; S.dwdz = T[2][2]; S.lw = T[2][3];
+    S.dhdx = T[3][0]; S.dhdy = T[3][1]; S.dhdz = T[3][2]; S.lh = T[3][3];
+    return S;
+}
+
 // Stencil sums of wave slice wid's two nodes from an item's LDS image, then
 // the pointwise RHS and the stage update of every (node, member) lane.
-template <int STAGE, bool EXACT, int NNBR, int DEPTH>
+template <int STAGE, bool EXACT, int NNBR, int DEPTH, bool MFMA = false>
 __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &tl,
                                              const DevStage &st, long B, const double *img,
                                              const TmemItem &I, const TmemOwn &O, int wid,
@@ -1306,6 +1397,9 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
         rw23[j % DEPTH] = wrow[2 * j + 1];
     };
     Sums S = {};
+    if constexpr (MFMA) {
+        S = tmem_sums_mfma<NNBR>(img, wid, lane);
+    } else {
     static_for<DEPTH>(
         [&](auto jc) { issue(decltype(jc)::value, row_off(decltype(jc)::value)); });
     static_for<NNBR>([&](auto jc) {
@@ -1323,6 +1417,7 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
             issue(j + DEPTH, off);
         }
     });
+    }
     Geo g;
     g.x = gp[G_X]; g.y = gp[G_Y]; g.z = gp[G_Z]; g.f = gp[G_F]; g.ghm = gp[G_GHM];
     g.pu0 = gp[G_PU]; g.pu1 = gp[G_PU + 1]; g.pu2 = gp[G_PU + 2];
@@ -1388,9 +1483,10 @@ __device__ __forceinline__ TmemRange tmem_range(const DevTiles &tl, int nblk)
 // frees an image it issues the next item's loads into it, waits for them
 // and writes the header naming the item after.  A compute wave issues its
 // own epilogue loads (H, D) one item ahead, before its stencil loop.
-template <int STAGE, bool EXACT, int NNBR>
+template <int STAGE, bool EXACT, int NNBR, bool MFMA = false>
 __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, DevStage st)
 {
+    static_assert(!(MFMA && EXACT), "the matrix-core sums are fast mode only");
     static_assert(NNBR > 0 && NNBR <= 31, "node-major positions: 31 slots + the own row");
     extern __shared__ double lds[];   // two item images
     const TmemRange r = tmem_range(tl, st.nblk);
@@ -1438,8 +1534,9 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
             const TmemItem In = tmem_hdr_item(lds + b * kTmemItemDoubles, wid);
             const TmemOwn on = tmem_own_load<STAGE>(st, st.src, B, In, lane);
             if (I.live)
-                tmem_compute<STAGE, EXACT, NNBR, 4>(m, tl, st, B, lds + b * kTmemItemDoubles, I,
-                                                    o, wid, lane);
+                tmem_compute<STAGE, EXACT, NNBR, 4, MFMA>(m, tl, st, B,
+                                                          lds + b * kTmemItemDoubles, I, o,
+                                                          wid, lane);
             __syncthreads();
             I = In;
             o = on;
