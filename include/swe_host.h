@@ -3,8 +3,9 @@
  * reordering, cfdl/sfdl layouts and padding, the serial single-patch set-up
  * and the profiling report.  These restate the reference's host modules
  * (cited per function) with the same argument meaning, array layouts and
- * error behaviour, in FP64 only (the reference's RUN_32BIT switch off,
- * swe_code/main/include/swe_config.h:75-93).
+ * error behaviour.  Values are held as doubles; the *_ex readers with
+ * fp32 != 0 reproduce the reference's FP32 model type (RUN_32BIT,
+ * swe_code/main/include/swe_config.h:75-93) by rounding what they read.
  *
  * All arrays are malloc'ed by the library and owned by the caller
  * (release with swe_gsmd_free / free()).  Functions return 0 on success and
@@ -23,6 +24,10 @@ extern "C" {
 #define SWE_HOST_EARG (-3)     /* invalid argument                           */
 #define SWE_HOST_ENOMEM (-4)
 #define SWE_HOST_EGRAPH (-5)   /* stencil graph not connected (RCM)          */
+#define SWE_HOST_EFORMAT (-6)  /* not a NetCDF classic file (CDF-1/2/5), or a
+                                  text variable where numbers are read      */
+#define SWE_HOST_ENOVAR (-7)   /* a NetCDF variable the reader needs is absent */
+#define SWE_HOST_ERANGE (-8)   /* a value does not fit the int it is read into */
 
 /* Global static model data, GSMD_struct (swe_config.h:127-165), FP64. */
 typedef struct swe_gsmd {
@@ -78,6 +83,24 @@ int swe_get_fcs_bin(const char *path, double **H_100);
 int swe_get_state_bin_ex(const char *path, int which, int fp32, double **H);
 void swe_gsmd_free(swe_gsmd *g);
 
+/* ---- NetCDF input (swe_code/io/input.c:14-186, nc2bin.c:10-84) ---------- */
+/* The reference's USE_NCIO readers on NetCDF classic files (CDF-1, CDF-2 64-bit
+ * offset, CDF-5 64-bit data; NetCDF-4/HDF5 files are SWE_HOST_EFORMAT),
+ * without libnetcdf.  Variables: Nnodes, Nnbr (int), a, g, gh0, timestep,
+ * gamma, x, y, z, f, ghm [Nnodes], p_u, p_v, p_w, gradghm [Nnodes][3],
+ * DPx, DPy, DPz, L, idx [Nnodes][Nnbr] (unit sphere, 1-based), H and H_100
+ * [Nnodes][4], of any numeric type (converted as nc_get_var_double /
+ * nc_get_var_int convert).  get_GSMD_nc: the same scaling and 0-based ids as
+ * the binary reader; a variable of another size than Nnodes/Nnbr imply is
+ * SWE_HOST_EREAD. */
+int swe_get_gsmd_nc(const char *path, swe_gsmd *g);
+int swe_get_gsmd_nc_ex(const char *path, int fp32, swe_gsmd *g);
+/* get_ICs_nc / get_FCs_nc: which 0 = H, 1 = H_100. */
+int swe_get_state_nc_ex(const char *path, int which, int fp32, double **H);
+/* nc2bin: writes the binary input file of a NetCDF one (the weights read,
+ * scaled and unscaled again, the ids made 1-based, as nc2bin.c does). */
+int swe_nc2bin(const char *nc_path, const char *bin_path);
+
 /* ---- RCM (swe_code/rcm/rcm.c:9-173, reorder_nodes.c:8-87) ---------------- */
 /* rcm_mapping: new position -> old node id, malloc'ed [Nnodes]. */
 int swe_rcm_mapping(const swe_gsmd *g, int **mapping);
@@ -114,6 +137,10 @@ int swe_problem_load(const char *path, int use_rcm, const swe_layout *l,
 /* With fp32 != 0: the FP32 model type's reading (swe_get_gsmd_bin_ex). */
 int swe_problem_load_ex(const char *path, int use_rcm, const swe_layout *l, int fp32,
                         swe_problem *p);
+/* With netcdf != 0 the input is a NetCDF file (main.c:99-106,
+ * SWE_USE_NETCDF == 1). */
+int swe_problem_load_fmt(const char *path, int use_rcm, const swe_layout *l, int fp32,
+                         int netcdf, swe_problem *p);
 void swe_problem_free(swe_problem *p);
 
 /* ---- verify_output (main.c:439-477) --------------------------------------- */

@@ -85,7 +85,12 @@ def lib() -> C.CDLL:
                                        C.POINTER(ProblemStruct)]
         L.swe_problem_load_ex.argtypes = [C.c_char_p, C.c_int, C.POINTER(Layout), C.c_int,
                                           C.POINTER(ProblemStruct)]
+        L.swe_problem_load_fmt.argtypes = [C.c_char_p, C.c_int, C.POINTER(Layout), C.c_int,
+                                           C.c_int, C.POINTER(ProblemStruct)]
         L.swe_problem_free.argtypes = [C.POINTER(ProblemStruct)]
+        L.swe_get_gsmd_nc_ex.argtypes = [C.c_char_p, C.c_int, C.POINTER(Gsmd)]
+        L.swe_get_state_nc_ex.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(_dp)]
+        L.swe_nc2bin.argtypes = [C.c_char_p, C.c_char_p]
         L.swe_verify_output.argtypes = [C.POINTER(Patch), C.POINTER(Layout), _dp, _dp, C.c_double]
         L.swe_rcm_mapping.argtypes = [C.POINTER(Gsmd), C.POINTER(_ip)]
         L.swe_get_gsmd_bin.argtypes = [C.c_char_p, C.POINTER(Gsmd)]
@@ -101,6 +106,26 @@ def lib() -> C.CDLL:
         L.swe_process_profiling.restype = None
         _lib = L
     return _lib
+
+
+HOST_ERRORS = {-1: "EOPEN", -2: "EREAD", -3: "EARG", -4: "ENOMEM", -5: "EGRAPH",
+               -6: "EFORMAT", -7: "ENOVAR", -8: "ERANGE"}
+
+
+class HostError(RuntimeError):
+    """A negative SWE_HOST_E* code of the C host (include/swe_host.h)."""
+
+    def __init__(self, code: int, what: str):
+        self.code = int(code)
+        self.name = HOST_ERRORS.get(self.code, str(self.code))
+        super().__init__(f"{what} failed with SWE_HOST_{self.name} ({self.code})")
+
+
+def nc2bin(nc_path: str, bin_path: str) -> None:
+    """The reference's nc2bin (swe_code/io/nc2bin.c) through the C host."""
+    err = lib().swe_nc2bin(nc_path.encode(), bin_path.encode())
+    if err:
+        raise HostError(err, f"swe_nc2bin({nc_path})")
 
 
 def make_layout(name: str = "cfdl", simd: int = 4) -> Layout:
@@ -119,15 +144,17 @@ class Problem:
     C host prepared; the C allocation is released before returning)."""
 
     def __init__(self, path: str, use_rcm: bool = True, layout: str = "cfdl",
-                 simd: int = 4, fp32: bool = False):
+                 simd: int = 4, fp32: bool = False, netcdf: bool = False):
         self._s = ProblemStruct()
         self.layout_name = layout
         self.layout = make_layout(layout, simd)
         self.fp32 = bool(fp32)   # values rounded as the reference's FP32 build reads them
-        err = lib().swe_problem_load_ex(path.encode(), int(bool(use_rcm)),
-                                        C.byref(self.layout), int(self.fp32), C.byref(self._s))
+        self.netcdf = bool(netcdf)   # input is a NetCDF classic file (SWE_USE_NETCDF=1)
+        err = lib().swe_problem_load_fmt(path.encode(), int(bool(use_rcm)),
+                                         C.byref(self.layout), int(self.fp32),
+                                         int(self.netcdf), C.byref(self._s))
         if err:
-            raise RuntimeError(f"swe_problem_load({path}) failed with {err}")
+            raise HostError(err, f"swe_problem_load({path})")
         g, p = self._s.g, self._s.patch
         self.n_nodes = g.Nnodes
         self.padded = g.padded_Nnodes

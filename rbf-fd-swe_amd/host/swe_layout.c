@@ -250,15 +250,23 @@ int swe_problem_load(const char *path, int use_rcm, const swe_layout *l,
 int swe_problem_load_ex(const char *path, int use_rcm, const swe_layout *l, int fp32,
                         swe_problem *p)
 {
+    return swe_problem_load_fmt(path, use_rcm, l, fp32, 0, p);
+}
+
+int swe_problem_load_fmt(const char *path, int use_rcm, const swe_layout *l, int fp32,
+                         int netcdf, swe_problem *p)
+{
     memset(p, 0, sizeof(*p));
     if (!l || l->simd <= 0)
         return SWE_HOST_EARG;
     p->layout = *l;
-    int err = swe_get_gsmd_bin_ex(path, fp32, &p->g);
+    int err = netcdf ? swe_get_gsmd_nc_ex(path, fp32, &p->g) : swe_get_gsmd_bin_ex(path, fp32, &p->g);
     if (!err)
-        err = swe_get_state_bin_ex(path, 0, fp32, &p->H_init);
+        err = netcdf ? swe_get_state_nc_ex(path, 0, fp32, &p->H_init)
+                     : swe_get_state_bin_ex(path, 0, fp32, &p->H_init);
     if (!err)
-        err = swe_get_state_bin_ex(path, 1, fp32, &p->H_100);
+        err = netcdf ? swe_get_state_nc_ex(path, 1, fp32, &p->H_100)
+                     : swe_get_state_bin_ex(path, 1, fp32, &p->H_100);
     if (!err && use_rcm) {
         err = swe_rcm_mapping(&p->g, &p->mapping);
         if (!err)

@@ -25,9 +25,11 @@
  *                    reference pads H_init, matrix_transformations.c:20)
  * Init pipeline main.c:94-190, attempt loop :294-392, verify_output
  * :439-477 (every member against H_100), profiling report profiling.c.
- * Errors: a missing parameter or a NetCDF request prints and exits 1, an
- * unreadable input exits 2 (input.c CHECK_OPEN_ERR), a HIP error prints
- * the library's message and exits -1 (device_setup.c checkError).
+ * SWE_USE_NETCDF=1 reads a NetCDF classic input (input.c:14-186, the
+ * reference's USE_NCIO build) with the C host's own reader.
+ * Errors: a missing parameter prints and exits 1, an unreadable input
+ * exits 2 (input.c CHECK_OPEN_ERR, CHECK_ERR), a HIP error prints the
+ * library's message and exits -1 (device_setup.c checkError).
  */
 #include "swe_hip.h"
 #include "swe_host.h"
@@ -150,16 +152,13 @@ int main(void)
         fprintf(stderr, "swe_run: unknown SWE_LAYOUT %s\n", layout_name);
         return 1;
     }
-    if (use_netcdf == 1) {
-        printf("Executable was not built for NetCDF I/O compatibility.\n\n");
-        return 1;
-    }
     if (nsteps < 0 || nattempts < 1 || members < 1) {
         fprintf(stderr, "swe_run: bad SWE_NSTEPS/SWE_NATTEMPTS/SWE_MEMBERS\n");
         return 1;
     }
     printf("\n\n============================= Runtime Configurations =============================\n\n");
     printf("Input from file: yes\n\tInput File Path: %s\n\n", input);
+    printf("I/O Method: %s\n", use_netcdf == 1 ? "NETCDF" : "BINARY");
     printf("Simulation length (timesteps): %d\n", nsteps);
     printf("Number of attempts: %d\n", nattempts);
     printf("USE RCM Node Ordering: %s\n", use_rcm == 1 ? "yes" : "no");
@@ -172,9 +171,14 @@ int main(void)
         return 1;
     }
     swe_problem prob;
-    int err = swe_problem_load(input, use_rcm == 1, &lay, &prob);
+    int err = swe_problem_load_fmt(input, use_rcm == 1, &lay, 0, use_netcdf == 1, &prob);
     if (err == SWE_HOST_EOPEN) {
         printf("Error opening file %s\n", input);
+        return 2;
+    }
+    if (err && use_netcdf == 1 && err != SWE_HOST_ENOMEM && err != SWE_HOST_EGRAPH) {
+        /* the reference's NetCDF reader exits 2 on any library error (CHECK_ERR) */
+        printf("Error in input.c: %s: host error %d\n", input, err);
         return 2;
     }
     if (err) {

@@ -122,9 +122,12 @@ def test_driver_errors(pkg, tmp_path):
                SWE_USE_OCL="0")
     r = subprocess.run([exe], env=env, capture_output=True, text=True)
     assert r.returncode == 2 and "Error opening file" in r.stdout
-    env["SWE_USE_NETCDF"] = "1"
+    env["SWE_USE_NETCDF"] = "1"   # the NetCDF reader: unopenable, then not NetCDF
     r = subprocess.run([exe], env=env, capture_output=True, text=True)
-    assert r.returncode == 1 and "NetCDF" in r.stdout
+    assert r.returncode == 2 and "Error opening file" in r.stdout
+    env["SWE_INPUT_FILE"] = os.path.join(ROOT, "tests", "golden", "icos642_case.bin")
+    r = subprocess.run([exe], env=env, capture_output=True, text=True)
+    assert r.returncode == 2 and "Error in input.c" in r.stdout
 
 
 def test_serve_library_exports_header(pkg):

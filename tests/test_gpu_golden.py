@@ -83,6 +83,32 @@ def test_driver_history_and_known_answer(pkg, tmp_path, mode):
     assert np.array_equal(states[-1], raw.H_final)
 
 
+def test_driver_netcdf_input_known_answer(pkg, tmp_path):
+    """swe_run with SWE_USE_NETCDF=1 (the reference's USE_NCIO input path) on
+    the golden case written as a NetCDF classic file: VERIFICATION SUCCEEDED
+    on every attempt and the same history as from the binary file."""
+    import subprocess
+    from conftest import ROOT
+    from nc_cases import write_case_nc
+    nc = tmp_path / "case.nc"
+    write_case_nc(pkg.mesh.read_bin(CASE), nc, version=2)
+    exe = os.path.join(ROOT, "rbf-fd-swe_amd", "swe_run")
+    base = dict(os.environ, SWE_NUM_COMPUTE_UNITS="1", SWE_NSTEPS="100", SWE_NATTEMPTS="2",
+                SWE_USE_RCM="1", SWE_USE_OCL="0", SWE_LAYOUT="cfdl", SWE_HIP_MODE="steps",
+                SWE_HIP_EXACT="1", SWE_HIST_INT="25")
+    print("case: swe_run NetCDF input", flush=True)
+    hists = {}
+    for fmt, path in (("1", nc), ("0", CASE)):
+        hists[fmt] = tmp_path / f"hist{fmt}.bin"
+        r = subprocess.run([exe], env=dict(base, SWE_INPUT_FILE=str(path), SWE_USE_NETCDF=fmt,
+                                           SWE_OUTPUT_FILE=str(hists[fmt])),
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert r.stdout.count("VERIFICATION SUCCEEDED") == 2
+        assert ("NETCDF" if fmt == "1" else "BINARY") in r.stdout
+    assert hists["1"].read_bytes() == hists["0"].read_bytes()
+
+
 @pytest.mark.parametrize("mode", ["substep", "steps"])
 def test_driver_restart_continues_bitwise(pkg, tmp_path, mode):
     """Restart from a history snapshot (SWE_RESTART_FILE/STEP): the continued
