@@ -71,7 +71,8 @@ def parse_args():
     ap.add_argument("--nodes", type=int, default=40962)
     ap.add_argument("--members", type=int, default=1)
     ap.add_argument("--member-block", type=int, default=0)
-    ap.add_argument("--kernel", default="auto", choices=["auto", "quad", "lane", "quadw", "pair", "tile"])
+    ap.add_argument("--kernel", default="auto",
+                    choices=["auto", "quad", "lane", "quadw", "pair", "tile", "oct"])
     ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--order", default="hilbert", choices=["host", "morton", "hilbert"],
                     help="node order of the device copy (results are identical in any order; "
@@ -149,6 +150,10 @@ def run_reference_cpu(path: str, n_nodes: int, target_s: float, layout: str = "c
 
 def dominant_kernel(args):
     """Name of the stage kernel the options select (the one timed)."""
+    if args.fp32:
+        k = args.kernel if args.kernel != "auto" else \
+            ("lane" if args.exact or args.nodes > 262144 else "pair")
+        return "k_stage_lane32" if k == "lane" else "k_stage_split32"
     if args.members == 1:
         if args.kernel == "auto":
             return "k_stage_pair"
@@ -325,21 +330,29 @@ def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026, sample_every=250
                                "equal to their solo runs"}
 
 
+FP32_KERNELS = {"lane": "k_stage_lane32 (one lane per node)",
+                "pair": "k_stage_split32 (2 lanes per node split the stencil slots)",
+                "quad": "k_stage_split32 (4 lanes per node split the stencil slots)",
+                "oct": "k_stage_split32 (8 lanes per node split the stencil slots)"}
+
+
 def fp32_leg(pkg, torch, nodes, steps, warmup):
     """The headline workload in the reference's FP32 model type (its shipped
     precision, RUN_32BIT)."""
     import numpy as np
     prob = pkg.host.Problem(pkg.mesh.case_path(nodes, seed=1), use_rcm=True, layout="cfdl",
                             fp32=True)
-    with pkg.runtime.HipContext(prob, fp32=True) as ctx:
-        ctx.set_state(prob.H_init)
-        ms = timed_steps(torch, ctx, steps, warm=warmup)
-        finite = bool(np.isfinite(ctx.get_state()).all())
+    with pkg.runtime.HipContext(prob, fp32=True, kernel="lane") as base:
+        opts, _ = base.tune(members=1, steps=20)   # lanes per node: 1, 2, 4 or 8
+        with pkg.runtime.HipContext(prob, fp32=True, share=base, kernel=opts["kernel"]) as ctx:
+            ctx.set_state(prob.H_init)
+            ms = timed_steps(torch, ctx, steps, warm=warmup)
+            finite = bool(np.isfinite(ctx.get_state()).all())
     launch_us = ms * 1e3 / 4
     achieved = stage_bytes(prob.padded, 1, fp32=True) / (launch_us * 1e-6) / 1e9
     return {"workload": f"N={prob.n_nodes}, 1 member, {steps} RK4 steps, FP32 "
                         "(the reference's RUN_32BIT type)",
-            "kernel": "k_stage_lane32", "dtype": "f32",
+            "kernel": FP32_KERNELS[opts["kernel"]], "dtype": "f32",
             "value": prob.n_nodes * 4 / (ms * 1e-3) / 1e9, "unit": "GNode-RHS/s",
             "ms_per_step": ms,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -136,6 +136,14 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
                                    distinct neighbour states are staged in
                                    LDS (31-point stencils, else QUADW; use a
                                    HILBERT node order for compact tiles)     */
+#define SWE_HIP_KERNEL_OCT 6    /* FP32 single state: 8 lanes per node
+                                   splitting the stencil slots (fast mode)   */
+/* In SWE_HIP_FP32 contexts: LANE = one lane per node (the only mapping of
+ * exact mode); PAIR, QUAD and OCT = 2, 4 or 8 lanes per node, each summing
+ * every 2nd, 4th or 8th stencil slot, the partial sums added across the
+ * group (fast mode); AUTO = LANE in exact mode, and in fast mode PAIR up to
+ * 262144 nodes (one lane per node would leave the chip under-filled), LANE
+ * above.  QUADW and TILE are FP64 mappings. */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 #define SWE_HIP_ORDER_HILBERT 2 /* device node order = 3-D Hilbert curve     */
@@ -198,8 +206,10 @@ int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int st
  *     loaded before the tile barrier);
  *   members > 1: every member block, untiled and LDS-tiled, and for member
  *     block 32 the member-lane tile shapes (tile_nodes 8 or 6 x tile_rows 70
- *     or 56).
- * FP64 contexts.  No counterpart in the reference. */
+ *     or 56);
+ *   SWE_HIP_FP32 contexts (flags must carry SWE_HIP_FP32, members == 1):
+ *     LANE, and in fast mode PAIR, QUAD and OCT (lanes per node).
+ * No counterpart in the reference. */
 int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
                     swe_hip_options *best, double *best_ms_per_step);
 

@@ -1,13 +1,32 @@
-// Launchers of the FP32 single-state stage kernel.
+// Launchers of the FP32 single-state stage kernels.
 #include "swe_ctx.h"
 
 namespace swe {
 namespace {
 
+template <int STAGE, int S>
+void launch_split32(const swe_hip_ctx *c, const DevMesh32 &m, DevStage32 st)
+{
+    st.nblk = blocks_x(c, (long)c->n * S, kSingleBlock);
+    const dim3 g(st.nblk), b(kSingleBlock);
+    if (c->nnbr == 31)
+        k_stage_split32<STAGE, 31, S><<<g, b, 0, c->stream>>>(m, st);
+    else
+        k_stage_split32<STAGE, 0, S><<<g, b, 0, c->stream>>>(m, st);
+}
+
 template <int STAGE, bool EX>
 void launch32(const swe_hip_ctx *c, DevStage32 st)
 {
     const DevMesh32 m = mesh32_of(c);
+    if constexpr (!EX) {   // fast mode: S lanes per node split the slots
+        switch (c->split32) {
+        case 2: launch_split32<STAGE, 2>(c, m, st); return;
+        case 4: launch_split32<STAGE, 4>(c, m, st); return;
+        case 8: launch_split32<STAGE, 8>(c, m, st); return;
+        default: break;
+        }
+    }
     // one-wave workgroups: at N = 40962 one lane per node is 641 waves, which
     // 256-thread workgroups would put on 161 of the 256 CUs
     st.nblk = blocks_x(c, (long)c->n, kSingleBlock);

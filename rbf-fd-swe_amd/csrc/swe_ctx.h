@@ -86,6 +86,7 @@ struct swe_hip_ctx {
     int sblk = 0;
     int B = 1, MB = 1;
     int kernel = SWE_HIP_KERNEL_QUAD;   // single-state mapping
+    int split32 = 1;                    // FP32: lanes per node (1, 2, 4, 8)
     int prefetch = 0;                   // stencil slots per load batch (0, 4, 8)
     int tile_nodes = 0, tile_rows = 0;  // tile shape of the tiled kernels (0: default)
     int ncu = 0;                        // compute units of the device

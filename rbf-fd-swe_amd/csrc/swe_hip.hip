@@ -762,7 +762,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "static data is NULL");
     if (o->members < 1 || o->members > 4096)
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
-    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_TILE)
+    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_OCT)
         return fail(SWE_HIP_EINVAL, "kernel must be one of SWE_HIP_KERNEL_*");
     if (o->tiled < 0 || o->tiled > 1)
         return fail(SWE_HIP_EINVAL, "tiled must be 0 or 1");
@@ -780,6 +780,14 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
                                     "tiled)");
     if (o->share && o->share->fp32 != fp32)
         return fail(SWE_HIP_EINVAL, "share: the shared context has the other model precision");
+    if (fp32 && (o->kernel == SWE_HIP_KERNEL_QUADW || o->kernel == SWE_HIP_KERNEL_TILE))
+        return fail(SWE_HIP_EINVAL, "SWE_HIP_FP32 kernels are AUTO, LANE, PAIR, QUAD and OCT");
+    if (fp32 && (o->flags & SWE_HIP_EXACT) && o->kernel != SWE_HIP_KERNEL_AUTO
+        && o->kernel != SWE_HIP_KERNEL_LANE)
+        return fail(SWE_HIP_EINVAL, "exact SWE_HIP_FP32 contexts sum each stencil in order: "
+                                    "kernel AUTO or LANE");
+    if (!fp32 && o->kernel == SWE_HIP_KERNEL_OCT)
+        return fail(SWE_HIP_EINVAL, "SWE_HIP_KERNEL_OCT is an FP32 mapping");
     if (o->members == 1 && o->tile_nodes != 0 && o->tile_nodes != 64 && o->tile_nodes != 128
         && o->tile_nodes != 256)
         return fail(SWE_HIP_EINVAL, "tile_nodes of the single-state tile kernel must be 0, 64, "
@@ -849,6 +857,19 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     else
         c->kernel = o->kernel;
     c->fp32 = fp32;
+    if (fp32) {
+        // lanes per node of the FP32 kernels; auto in fast mode: two lanes
+        // per node below 262144 nodes, where one lane per node leaves fewer
+        // than 4 waves per SIMD (N = 40962: 5.7 against 10.8 us per launch),
+        // one lane above (N = 655362: 108.5 against 112.8 us)
+        const int fast_auto = c->n <= 262144 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_LANE;
+        const int k = o->kernel == SWE_HIP_KERNEL_AUTO
+                          ? ((o->flags & SWE_HIP_EXACT) ? SWE_HIP_KERNEL_LANE : fast_auto)
+                          : o->kernel;
+        c->kernel = k;
+        c->split32 = k == SWE_HIP_KERNEL_PAIR ? 2 : k == SWE_HIP_KERNEL_QUAD ? 4
+                   : k == SWE_HIP_KERNEL_OCT ? 8 : 1;
+    }
     c->prefetch = o->prefetch;
     c->tile_nodes = o->tile_nodes;
     c->tile_rows = o->tile_rows;
@@ -990,14 +1011,21 @@ int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int st
 int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
                     swe_hip_options *best, double *best_ms)
 {
-    if (!base || members < 1 || steps < 1 || !best || (flags & SWE_HIP_FP32) || base->fp32)
-        return fail(SWE_HIP_EINVAL, "tune_ex: bad arguments (FP64 contexts)");
+    if (!base || members < 1 || steps < 1 || !best
+        || ((flags & SWE_HIP_FP32) != 0) != base->fp32 || (base->fp32 && members != 1))
+        return fail(SWE_HIP_EINVAL, "tune_ex: bad arguments (flags must carry base's "
+                                    "precision; FP32 contexts hold one member)");
     std::vector<swe_hip_options> cand;
     auto add = [&](int mb, int kernel, int prefetch, int tiled, int tn, int tr) {
         cand.push_back(swe_hip_options{members, flags, base->dev, mb, base, kernel, prefetch, 0,
                                        tiled, 0, tn, tr});
     };
-    if (members == 1) {
+    if (base->fp32) {   // lanes per node; exact mode sums every stencil in one lane
+        add(0, SWE_HIP_KERNEL_LANE, 0, 0, 0, 0);
+        if (!(flags & SWE_HIP_EXACT))
+            for (int k : {SWE_HIP_KERNEL_PAIR, SWE_HIP_KERNEL_QUAD, SWE_HIP_KERNEL_OCT})
+                add(0, k, 0, 0, 0, 0);
+    } else if (members == 1) {
         add(0, SWE_HIP_KERNEL_PAIR, 0, 0, 0, 0);
         if (base->nnbr == 31)
             for (int tn : {64, 128, 256})

@@ -182,4 +182,98 @@ __global__ __launch_bounds__(256) void k_stage_lane32(DevMesh32 m, DevStage32 st
         stage_update32<STAGE, EXACT>(st, 4L * node, F);
 }
 
+
+// A float moved across lanes by DPP (ctrl: quad_perm or row_half_mirror).
+template <int CTRL>
+__device__ __forceinline__ float dpp32(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Adds the partial sums of the S lanes of a group (S = 2, 4 or 8 aligned
+// lanes): pairs, then quads (quad_perm), then halves of 8 (row_half_mirror).
+// Every round adds two partial sums in one lane and the same two in its
+// partner (x + y and y + x round alike), so all lanes of a group end with
+// the same total.
+template <int S>
+__device__ __forceinline__ float group_sum32(float x)
+{
+    x += dpp32<0xB1>(x);            // quad_perm [1, 0, 3, 2]
+    if constexpr (S >= 4)
+        x += dpp32<0x4E>(x);        // quad_perm [2, 3, 0, 1]
+    if constexpr (S >= 8)
+        x += dpp32<0x141>(x);       // row_half_mirror: lane i <-> 7 - i within 8
+    return x;
+}
+
+// Single state, S lanes per node splitting the stencil slots (fast mode):
+// lane h of a node's group sums slots h, h + S, ... in order with FMAs, the
+// group adds its S partial sums (group_sum32), and the group's first lane
+// evaluates the RHS and writes the stage update.  S times the waves of
+// k_stage_lane32 with S times fewer slots each, for meshes too small to fill
+// the chip with one lane per node; the same loads per node (one 16-byte
+// weight load, one 16-byte gather and one id per slot).
+template <int STAGE, int NNBR, int S>
+__global__ __launch_bounds__(256) void k_stage_split32(DevMesh32 m, DevStage32 st)
+{
+    static_assert(S == 2 || S == 4 || S == 8, "groups of 2, 4 or 8 lanes");
+    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
+    const int h = (int)threadIdx.x & (S - 1);
+    const bool live = t / S < m.n;
+    const int node = live ? t / S : m.n - 1;
+    const int nn = NNBR > 0 ? NNBR : m.nnbr;
+    const long ld = m.sblk ? 16L : (long)m.ld;
+    const long W = NNBR > 0 ? NNBR : m.nnbr;
+    const long sb = m.sblk ? ((long)(node >> 4) * W << 4) + (node & 15) : (long)node;
+    const int *__restrict__ nb = m.nbr + sb;
+    const float *__restrict__ wp = m.wts + 4L * sb;
+    const float *__restrict__ src = st.src;
+    Sums32 P = {};
+    auto slot = [&](int j) {
+        const int q = nb[j * ld];
+        const float4 w = *reinterpret_cast<const float4 *>(wp + 4L * j * ld);
+        const float4 v = *reinterpret_cast<const float4 *>(src + 4L * q);
+        P.dudx = madd32<false>(P.dudx, w.x, v.x); P.dvdx = madd32<false>(P.dvdx, w.x, v.y);
+        P.dwdx = madd32<false>(P.dwdx, w.x, v.z); P.dhdx = madd32<false>(P.dhdx, w.x, v.w);
+        P.dudy = madd32<false>(P.dudy, w.y, v.x); P.dvdy = madd32<false>(P.dvdy, w.y, v.y);
+        P.dwdy = madd32<false>(P.dwdy, w.y, v.z); P.dhdy = madd32<false>(P.dhdy, w.y, v.w);
+        P.dudz = madd32<false>(P.dudz, w.z, v.x); P.dvdz = madd32<false>(P.dvdz, w.z, v.y);
+        P.dwdz = madd32<false>(P.dwdz, w.z, v.z); P.dhdz = madd32<false>(P.dhdz, w.z, v.w);
+        P.lu = madd32<false>(P.lu, w.w, v.x); P.lv = madd32<false>(P.lv, w.w, v.y);
+        P.lw = madd32<false>(P.lw, w.w, v.z); P.lh = madd32<false>(P.lh, w.w, v.w);
+    };
+    if constexpr (NNBR > 0) {
+        constexpr int K = (NNBR + S - 1) / S;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int j = h + S * k;
+            if (NNBR % S == 0 || j < NNBR)
+                slot(j);
+        }
+    } else {
+#pragma unroll 2
+        for (int j = h; j < nn; j += S)
+            slot(j);
+    }
+    Sums32 T;
+    T.dudx = group_sum32<S>(P.dudx); T.dvdx = group_sum32<S>(P.dvdx);
+    T.dwdx = group_sum32<S>(P.dwdx); T.dhdx = group_sum32<S>(P.dhdx);
+    T.dudy = group_sum32<S>(P.dudy); T.dvdy = group_sum32<S>(P.dvdy);
+    T.dwdy = group_sum32<S>(P.dwdy); T.dhdy = group_sum32<S>(P.dhdy);
+    T.dudz = group_sum32<S>(P.dudz); T.dvdz = group_sum32<S>(P.dvdz);
+    T.dwdz = group_sum32<S>(P.dwdz); T.dhdz = group_sum32<S>(P.dhdz);
+    T.lu = group_sum32<S>(P.lu); T.lv = group_sum32<S>(P.lv);
+    T.lw = group_sum32<S>(P.lw); T.lh = group_sum32<S>(P.lh);
+    if (h != 0 || !live)
+        return;
+    const float4 own = *reinterpret_cast<const float4 *>(src + 4L * node);
+    float g[GEO_N];
+#pragma unroll
+    for (int f = 0; f < GEO_N; ++f)
+        g[f] = m.geo[(long)f * m.ld + node];
+    float F[4];
+    rhs_point32(T, own.x, own.y, own.z, own.w, g, m.gh0, F);
+    stage_update32<STAGE, false>(st, 4L * node, F);
+}
+
 }  // namespace swe

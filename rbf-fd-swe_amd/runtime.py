@@ -17,7 +17,7 @@ _ip = C.POINTER(C.c_int)
 SWE_HIP_EXACT = 1
 SWE_HIP_FP32 = 2
 _fp = C.POINTER(C.c_float)
-KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4, "tile": 5}
+KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4, "tile": 5, "oct": 6}
 ORDERS = {"host": 0, "morton": 1, "hilbert": 2}
 
 # Every function include/swe_hip.h declares.
@@ -200,7 +200,7 @@ class HipContext:
         self.exact = bool(exact)
         self.fp32 = bool(fp32)
         self.n_state = 4 * prob.patch_size
-        if self.fp32:
+        if self.fp32 and kernel == "auto" and share is None:   # the FP32 drop-in entry point
             s, keep = static32_from_problem(prob)
             h = C.c_void_p()
             _chk(lib().swe_hip_create32(C.byref(s), self.members,
@@ -209,7 +209,10 @@ class HipContext:
             self._share = None
             self._h = h
             return
-        opts = SweHipOptions(self.members, SWE_HIP_EXACT if exact else 0, device,
+        # FP32 with options: the double entry point with SWE_HIP_FP32 (the
+        # problem's values are already the FP32 type's, exactly representable)
+        flags = (SWE_HIP_EXACT if exact else 0) | (SWE_HIP_FP32 if self.fp32 else 0)
+        opts = SweHipOptions(self.members, flags, device,
                              int(member_block), share.handle if share is not None else None,
                              KERNELS[kernel], int(prefetch), ORDERS[order], int(tiled),
                              int(slot_block), int(tile_nodes), int(tile_rows))
@@ -337,8 +340,9 @@ class HipContext:
         the fastest configuration swe_hip_tune_ex finds on this mesh."""
         o = SweHipOptions()
         ms = C.c_double()
-        _chk(lib().swe_hip_tune_ex(self.handle, int(members), SWE_HIP_EXACT if exact else 0,
-                                   int(steps), C.byref(o), C.byref(ms)), "swe_hip_tune_ex")
+        flags = (SWE_HIP_EXACT if exact else 0) | (SWE_HIP_FP32 if self.fp32 else 0)
+        _chk(lib().swe_hip_tune_ex(self.handle, int(members), flags, int(steps), C.byref(o),
+                                   C.byref(ms)), "swe_hip_tune_ex")
         kname = {v: k for k, v in KERNELS.items()}[o.kernel]
         opts = dict(members=o.members, member_block=o.member_block, kernel=kname,
                     prefetch=o.prefetch, tiled=o.tiled, tile_nodes=o.tile_nodes,

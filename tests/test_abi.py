@@ -75,7 +75,7 @@ def test_bad_options_rejected_before_device(pkg):
     L = pkg.runtime.lib()
     base = dict(members=1, flags=0, device=0, member_block=0, share=None, kernel=0,
                 prefetch=0, node_order=0, tiled=0, slot_block=0)
-    for field, bad in [("kernel", 6), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
+    for field, bad in [("kernel", 7), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
                        ("slot_block", 8), ("members", 0), ("member_block", 3),
                        ("tiled", 2), ("tiled", -1), ("tile_nodes", 100), ("tile_nodes", -2),
                        ("tile_rows", 8), ("tile_rows", 71)]:
@@ -85,6 +85,22 @@ def test_bad_options_rejected_before_device(pkg):
         out = C.c_void_p()
         assert L.swe_hip_create_ex(C.byref(s), C.byref(o), C.byref(out)) == -1, field
         assert out.value is None
+
+
+def test_fp32_kernel_options_rejected_before_device(pkg):
+    """Exact FP32 contexts sum every stencil in one lane; OCT is FP32-only;
+    QUADW and TILE are FP64 mappings."""
+    path = pkg.mesh.case_path(642, seed=1)
+    p32 = pkg.host.Problem(path, use_rcm=True, layout="cfdl", fp32=True)
+    p64 = pkg.host.Problem(path, use_rcm=True, layout="cfdl")
+    bad = [(p32, dict(fp32=True, exact=True, kernel="pair")),
+           (p32, dict(fp32=True, exact=True, kernel="quad")),
+           (p32, dict(fp32=True, kernel="quadw")), (p32, dict(fp32=True, kernel="tile")),
+           (p64, dict(kernel="oct"))]
+    for prob, kw in bad:
+        with pytest.raises(pkg.runtime.SweHipError) as ei:
+            pkg.runtime.HipContext(prob, **kw)
+        assert ei.value.code == -1, kw
 
 
 def test_no_device_is_a_loud_error(pkg):

@@ -59,13 +59,18 @@ def test_fp32_exact_vs_oracle(pkg, pyoracle, n, layout):
         assert np.array_equal(ctx.get_state(), orc.run(prob.H_init, 20))
 
 
-@pytest.mark.parametrize("n,steps", [(642, 100), (40962, 1000)])
-def test_fp32_fast_known_answer(pkg, pyoracle, n, steps):
+@pytest.mark.parametrize("n,steps,kernel", [(642, 100, "auto"), (642, 100, "lane"),
+                                            (642, 100, "quad"), (642, 100, "oct"),
+                                            (40962, 1000, "auto"), (40962, 1000, "lane")])
+def test_fp32_fast_known_answer(pkg, pyoracle, n, steps, kernel):
+    """Fast FP32 kernels (one lane per node; 2 (auto below 262144 nodes), 4 or
+    8 lanes per node splitting the stencil slots) within 1e-3 of the strict
+    FP32 oracle."""
     path = CASE if n == 642 else pkg.mesh.case_path(n, seed=1)
     prob = pkg.host.Problem(path, use_rcm=True, layout="cfdl", fp32=True)
-    print(f"case: FP32 fast N={n} {steps} steps", flush=True)
+    print(f"case: FP32 fast N={n} {steps} steps kernel {kernel}", flush=True)
     want = pyoracle.Oracle(prob, "strict32").run(prob.H_init, steps)
-    with pkg.runtime.HipContext(prob, fp32=True) as ctx:
+    with pkg.runtime.HipContext(prob, fp32=True, kernel=kernel) as ctx:
         ctx.set_state(prob.H_init)
         ctx.rk_steps(steps)
         got = ctx.get_state()
@@ -87,3 +92,25 @@ def test_fp32_entry_points_reject_the_other_precision(pkg):
         assert L.swe_hip_set_state32(c64.handle, 0, H32.ctypes.data_as(pkg.runtime._fp)) == -1
         with pytest.raises(pkg.runtime.SweHipError):
             c32.rk_steps_noalloc(c32.state_device_ptr(), 1)
+
+
+def test_fp32_tuner_and_split_kernels_agree(pkg):
+    """swe_hip_tune_ex over the FP32 mappings returns a fast-mode kernel, and
+    every split mapping gives the same state after 10 steps (the group sums
+    are formed alike in every lane; the mappings differ only in the order of
+    the partial sums, so within 1e-4 of one another, FP32 rounding)."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl",
+                            fp32=True)
+    print("case: FP32 tuner N=10242", flush=True)
+    with pkg.runtime.HipContext(prob, fp32=True, kernel="lane") as base:
+        opts, ms = base.tune(members=1, steps=3)
+        assert opts["kernel"] in ("lane", "pair", "quad", "oct") and ms > 0
+        out = {}
+        for k in ("lane", "pair", "quad", "oct"):
+            with pkg.runtime.HipContext(prob, fp32=True, share=base, kernel=k) as ctx:
+                ctx.set_state(prob.H_init)
+                ctx.rk_steps(10)
+                out[k] = ctx.get_state().astype(np.float64)
+    for k in ("pair", "quad", "oct"):
+        err = rel_by_var(out[k], out["lane"], prob)
+        assert (err <= 1e-4).all(), (k, err)
