@@ -1436,6 +1436,10 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
         ou = r[0]; ov = r[32]; ow = r[64]; oh = r[96];
     } else {
         ou = st.src[e]; ov = st.src[e + B]; ow = st.src[e + 2 * B]; oh = st.src[e + 3 * B];
+        // keeps the two branches apart: merged, their loads become flat loads
+        // of a selected pointer, and a flat load waits for every outstanding
+        // global access of the wave (the previous item's stores included)
+        asm volatile("" : "+v"(ou), "+v"(ov), "+v"(ow), "+v"(oh));
     }
     double F[4];
     rhs_point(S, ou, ov, ow, oh, g, m.gh0, F);
