@@ -1016,9 +1016,13 @@ __global__ __launch_bounds__(2 * TN) void k_stage_ptile(DevMesh m, DevTiles tl, 
     S.dhdx = k ? bx : pbx; S.dhdy = k ? by : pby; S.dhdz = k ? bz : pbz; S.lh = k ? bl : pbl;
     // own row: position in the spare slot 31 (lane 1's last dword, high half)
     const unsigned ownw = (unsigned)pair_bcast_i<1>((int)pw[7]) >> 16;
-    const double2 own = ownw != 0xffffu
-                            ? rows[2 * (int)ownw + k]
-                            : *reinterpret_cast<const double2 *>(src + 4L * node + 2 * k);
+    double2 own;
+    if (ownw != 0xffffu) {
+        own = rows[2 * (int)ownw + k];
+    } else {
+        own = *reinterpret_cast<const double2 *>(src + 4L * node + 2 * k);
+        asm volatile("" : "+v"(own.x), "+v"(own.y));   // no merged flat load (see tmem_compute)
+    }
     const double po0 = pair_swap(own.x), po1 = pair_swap(own.y);
     const double u = k ? po0 : own.x, v = k ? po1 : own.y;
     const double w = k ? own.x : po0, h = k ? own.y : po1;
