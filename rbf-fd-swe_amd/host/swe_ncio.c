@@ -51,6 +51,7 @@ typedef struct ncvar {
 
 typedef struct ncfile {
     FILE *fp;
+    long long fsize;   /* bytes in the file: bounds every count of the header */
     int version;
     long long numrecs, recsize;
     long long ndims, nvars;
@@ -116,6 +117,9 @@ static int skip(FILE *fp, long long n)
 
 static long long pad4(long long n) { return (n + 3) & ~3LL; }
 
+/* Byte counts of a file are below 2^62: anything larger is a malformed header. */
+#define NC_MAX_BYTES (1LL << 62)
+
 static int rd_name(ncfile *nc, char **out)
 {
     long long len;
@@ -166,7 +170,7 @@ static int skip_atts(ncfile *nc)
             err = rd_u32(nc, &t);
         if (!err)
             err = rd_count(nc, nc->version == 5, &cnt);
-        if (!err && (type_size((int)t) == 0 || cnt < 0))
+        if (!err && (type_size((int)t) == 0 || cnt < 0 || cnt > NC_MAX_BYTES / 8))
             err = SWE_HOST_EFORMAT;
         if (!err)
             err = skip(nc->fp, pad4(cnt * type_size((int)t)));
@@ -193,6 +197,11 @@ static int nc_open_file(const char *path, ncfile *nc)
     nc->fp = fopen(path, "rb");
     if (!nc->fp)
         return SWE_HOST_EOPEN;
+    if (fseeko(nc->fp, 0, SEEK_END) != 0 || (nc->fsize = (long long)ftello(nc->fp)) < 0
+        || fseeko(nc->fp, 0, SEEK_SET) != 0) {
+        nc_close_file(nc);
+        return SWE_HOST_EREAD;
+    }
     unsigned char magic[8];
     int err = rd(nc->fp, magic, 4);
     if (!err && memcmp(magic, "\x89HDF", 4) == 0)
@@ -213,9 +222,15 @@ static int nc_open_file(const char *path, ncfile *nc)
     }
     numrecs = be(magic, wide ? 8 : 4);
     const int streaming = wide ? numrecs == UINT64_MAX : numrecs == 0xFFFFFFFFu;
+    if (!streaming && numrecs > (uint64_t)NC_MAX_BYTES) {
+        nc_close_file(nc);
+        return SWE_HOST_EFORMAT;
+    }
     nc->numrecs = streaming ? -1 : (long long)numrecs;
     /* dimensions */
     err = rd_list_head(nc, TAG_DIM, &nc->ndims);
+    if (!err && nc->ndims > nc->fsize / 8)   /* a dimension takes at least 8 bytes */
+        err = SWE_HOST_EFORMAT;
     if (!err && nc->ndims > 0) {
         nc->dimlen = (long long *)calloc((size_t)nc->ndims, sizeof(long long));
         if (!nc->dimlen)
@@ -234,6 +249,8 @@ static int nc_open_file(const char *path, ncfile *nc)
         err = skip_atts(nc);   /* global attributes */
     if (!err)
         err = rd_list_head(nc, TAG_VAR, &nc->nvars);
+    if (!err && nc->nvars > nc->fsize / 16)   /* a variable takes at least 16 bytes */
+        err = SWE_HOST_EFORMAT;
     if (!err && nc->nvars > 0) {
         nc->vars = (ncvar *)calloc((size_t)nc->nvars, sizeof(ncvar));
         if (!nc->vars)
@@ -245,7 +262,7 @@ static int nc_open_file(const char *path, ncfile *nc)
         long long nd = 0;
         if (!err)
             err = rd_count(nc, wide, &nd);
-        if (!err && (nd < 0 || nd > 1024))
+        if (!err && (nd < 0 || nd > 1024 || nd > nc->fsize / 4))
             err = SWE_HOST_EFORMAT;
         if (!err) {
             x->ndims = (int)nd;
@@ -265,6 +282,8 @@ static int nc_open_file(const char *path, ncfile *nc)
                 if (k != 0)   /* only the first dimension may be the record one */
                     err = SWE_HOST_EFORMAT;
                 x->isrec = 1;
+            } else if (x->nelems > NC_MAX_BYTES / 8 / len) {
+                err = SWE_HOST_EFORMAT;   /* element count past any file size */
             } else {
                 x->nelems *= len;
             }
@@ -281,7 +300,8 @@ static int nc_open_file(const char *path, ncfile *nc)
             err = rd_count(nc, wide, &x->vsize);
         if (!err)
             err = rd_count(nc, nc->version != 1, &x->begin);
-        if (!err && x->begin < 0)
+        if (!err && (x->begin < 0 || x->begin > NC_MAX_BYTES || x->vsize < 0
+                     || x->vsize > NC_MAX_BYTES))
             err = SWE_HOST_EFORMAT;
     }
     if (err) {
@@ -294,7 +314,11 @@ static int nc_open_file(const char *path, ncfile *nc)
     for (long long v = 0; v < nc->nvars; v++)
         if (nc->vars[v].isrec) {
             nrec++;
-            nc->recsize += nc->vars[v].vsize;
+            nc->recsize += nc->vars[v].vsize;   /* each < 2^62, at most 2^31 of them */
+            if (nc->recsize > NC_MAX_BYTES) {
+                nc_close_file(nc);
+                return SWE_HOST_EFORMAT;
+            }
         }
     if (nrec == 1)
         for (long long v = 0; v < nc->nvars; v++)
@@ -322,9 +346,14 @@ static const ncvar *nc_find(const ncfile *nc, const char *name)
     return NULL;
 }
 
+/* Elements of x in the file; -1 when the count is past any file size. */
 static long long var_count(const ncfile *nc, const ncvar *x)
 {
-    return x->isrec ? x->nelems * nc->numrecs : x->nelems;
+    if (!x->isrec)
+        return x->nelems;
+    if (nc->numrecs > 0 && x->nelems > NC_MAX_BYTES / nc->numrecs)
+        return -1;
+    return x->nelems * nc->numrecs;
 }
 
 /* Element k of a big-endian buffer of type t as a double (integers of more
@@ -384,12 +413,34 @@ static int conv_i32(const unsigned char *p, int t, void *out, long long k)
     return 0;
 }
 
-static int read_var(ncfile *nc, const ncvar *x, long long want, conv_fn conv, void *out)
+/* x holds exactly `want` numbers, all of them inside the file (checked
+ * before anything is allocated for them). */
+static int check_var(const ncfile *nc, const ncvar *x, long long want)
 {
     if (x->type == NC_CHAR)
         return SWE_HOST_EFORMAT;   /* NC_ECHAR: text is not numeric */
-    if (var_count(nc, x) != want)
+    if (want < 0 || var_count(nc, x) != want)
         return SWE_HOST_EREAD;
+    const long long es = type_size(x->type);
+    const long long run = x->isrec ? x->nelems * es : want * es;
+    if (x->isrec && nc->numrecs > 0
+        && (nc->recsize <= 0 || nc->numrecs - 1 > nc->fsize / nc->recsize))
+        return SWE_HOST_EREAD;   /* records past the end of the file */
+    const long long last = x->isrec && nc->numrecs > 0 ? (nc->numrecs - 1) * nc->recsize : 0;
+    if (x->begin + last > nc->fsize || run > nc->fsize - x->begin - last)
+        return SWE_HOST_EREAD;   /* the data run past the end of the file */
+    return 0;
+}
+
+static int read_var(ncfile *nc, const ncvar *x, long long want, conv_fn conv, void *out)
+{
+    const int bad = check_var(nc, x, want);
+    if (bad)
+        return bad;
+    /* the records must lie inside the representable file range */
+    if (x->isrec && nc->numrecs > 0
+        && (nc->recsize > 0 && nc->numrecs - 1 > (NC_MAX_BYTES - x->begin) / nc->recsize))
+        return SWE_HOST_EFORMAT;
     const int es = type_size(x->type);
     const long long per = x->isrec ? x->nelems : want;   /* contiguous elements per run */
     const long long runs = x->isrec ? nc->numrecs : 1;
@@ -426,6 +477,9 @@ static int get_f64(ncfile *nc, const char *name, long long count, double **out)
     const ncvar *x = nc_find(nc, name);
     if (!x)
         return SWE_HOST_ENOVAR;
+    const int bad = check_var(nc, x, count);
+    if (bad)
+        return bad;
     double *a = (double *)malloc(sizeof(double) * (size_t)(count ? count : 1));
     if (!a)
         return SWE_HOST_ENOMEM;
@@ -496,6 +550,8 @@ int swe_get_gsmd_nc_ex(const char *path, int fp32, swe_gsmd *g)
         const ncvar *x = nc_find(&nc, "idx");
         if (!x)
             err = SWE_HOST_ENOVAR;
+        else if ((err = check_var(&nc, x, NW)) != 0)
+            ;
         else if (!(g->idx = (int *)malloc(sizeof(int) * (size_t)NW)))
             err = SWE_HOST_ENOMEM;
         else

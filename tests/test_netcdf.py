@@ -164,3 +164,53 @@ def _set_var(path, name, fn):
     with netcdf_file(str(path), "a") as f:
         v = f.variables[name]
         v[...] = fn(v[...].copy())
+
+
+def test_malformed_files_never_crash_the_reader(pkg, raw, tmp_path):
+    """The reader under AddressSanitizer and UBSan (a host-only build of the
+    converter program): headers with flipped bytes, extreme counts, sizes and
+    offsets, and truncated files give an error code (or a file that still
+    reads), never an invalid access, overflow or crash."""
+    import random
+    import shutil
+    import struct
+    if shutil.which("gcc") is None:
+        pytest.skip("no host compiler")
+    host = os.path.join(ROOT, "rbf-fd-swe_amd", "host")
+    exe = tmp_path / "nc2bin_asan"
+    srcs = [os.path.join(host, f) for f in ("swe_nc2bin.c", "swe_ncio.c", "swe_io.c",
+                                             "swe_rcm.c", "swe_layout.c", "swe_profiling.c")]
+    subprocess.run(["gcc", "-std=gnu99", "-O1", "-g", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", "-I", os.path.join(ROOT, "include"),
+                    "-o", str(exe)] + srcs + ["-lm"], check=True)
+    good = tmp_path / "good.nc"
+    write_case_nc(raw, good, version=2)
+    base = bytearray(good.read_bytes())
+    hdr = 1200   # the header of this file (dimensions, variables, offsets)
+    rng = random.Random(20261203)
+    extremes = [0, 1, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFF, 0xFFFFFFFE, 0x40000000, 3]
+    cases = []
+    for n in range(160):
+        b = bytearray(base)
+        kind = n % 4
+        if kind == 0:      # flipped bytes
+            for _ in range(rng.randint(1, 6)):
+                b[rng.randrange(hdr)] ^= 1 << rng.randrange(8)
+        elif kind == 1:    # an extreme 32-bit field at a word boundary
+            off = 4 * rng.randrange(hdr // 4)
+            b[off:off + 4] = struct.pack(">I", rng.choice(extremes))
+        elif kind == 2:    # an extreme 64-bit field (offsets of CDF-2)
+            off = 4 * rng.randrange(hdr // 4 - 1)
+            b[off:off + 8] = struct.pack(">Q", rng.choice(
+                [0, 2**62, 2**63 - 1, 2**63, 2**64 - 1, len(base) - 8]))
+        else:              # truncated
+            b = b[:rng.randrange(len(base))]
+        cases.append(bytes(b))
+    env = dict(os.environ, ASAN_OPTIONS="exitcode=86:detect_leaks=0",
+               UBSAN_OPTIONS="exitcode=87:print_stacktrace=1")
+    for k, data in enumerate(cases):
+        f = tmp_path / "m.nc"
+        f.write_bytes(data)
+        env.update(NC2BIN_INPUT_FILE=str(f), NC2BIN_OUTPUT_FILE=str(tmp_path / "m.bin"))
+        r = subprocess.run([str(exe)], env=env, capture_output=True, text=True, timeout=60)
+        assert r.returncode in (0, 1, 2), (k, r.returncode, r.stderr[-2000:])
