@@ -38,12 +38,23 @@ static double *read_f64(FILE *fp, size_t count, int *err)
     return a;
 }
 
+/* The header, checked against the file's size: the format's sections have
+ * fixed sizes once N and Nnbr are known, so a shorter file (or a corrupted
+ * count) is an error before anything is allocated for it. */
 static int read_header(FILE *fp, int *N, int *Nnbr)
 {
     int32_t hdr[2];
     if (read_exact(fp, hdr, sizeof(int32_t), 2))
         return SWE_HOST_EREAD;
     if (hdr[0] <= 0 || hdr[1] <= 0)
+        return SWE_HOST_EREAD;
+    const long long n = hdr[0], w = hdr[1];
+    const long long need = 4LL * (2 + n * w) + 8LL * (4 + 17 * n + 4 * n * w + 8 * n);
+    const long pos = ftell(fp);
+    if (pos < 0 || fseek(fp, 0, SEEK_END) != 0)
+        return SWE_HOST_EREAD;
+    const long size = ftell(fp);
+    if (size < 0 || (long long)size < need || fseek(fp, pos, SEEK_SET) != 0)
         return SWE_HOST_EREAD;
     *N = hdr[0];
     *Nnbr = hdr[1];
