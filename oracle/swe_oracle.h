@@ -8,13 +8,17 @@
  * cpu_baseline leg may load it; the product (libswe_hip.so and the C host)
  * never links or calls anything here.
  *
- * Pinning of each function (what proves the restatement right):
+ * Pinning of each function (what proves the restatement right; the tests
+ * are tests/test_reference_parity.py, tests/test_golden.py and
+ * tests/test_oracle_pinning.py):
  *   oracle_eval_rhs     - bitwise equal to the reference build (oracle/_ref)
  *                         on the shipped 2562 input and generated meshes;
- *                         dense brute-force RHS on tiny meshes; closed forms
- *                         (solid-body state, zero-weight stencils).
+ *                         against dense stencil operators on a 162-node mesh
+ *                         (zero-velocity and zero-depth states), the
+ *                         constant-state closed form and tangency.
  *   oracle_rk_substep   - bitwise equal to the reference build after 1/10/100
- *                         steps; textbook RK4 on a linear ODE.
+ *                         steps; one step of the linear system dH/dt = L H
+ *                         equals the textbook RK4 polynomial.
  *   oracle_verify       - known-answer H_100 of the shipped 2562 input
  *                         (count == 0 at 1e-9) and planted-error counts.
  */
