@@ -153,7 +153,7 @@ def dominant_kernel(args):
     if args.fp32:
         k = args.kernel if args.kernel != "auto" else \
             ("lane" if args.exact or args.nodes > 262144 else "pair")
-        return "k_stage_lane32" if k == "lane" else "k_stage_split32"
+        return {"lane": "k_stage_lane32", "tile": "k_stage_tile32"}.get(k, "k_stage_split32")
     if args.members == 1:
         if args.kernel == "auto":
             return "k_stage_pair"
@@ -333,7 +333,8 @@ def serve_leg(pkg, n_requests, rate, kernel="tiles", seed=2026, sample_every=250
 FP32_KERNELS = {"lane": "k_stage_lane32 (one lane per node)",
                 "pair": "k_stage_split32 (2 lanes per node split the stencil slots)",
                 "quad": "k_stage_split32 (4 lanes per node split the stencil slots)",
-                "oct": "k_stage_split32 (8 lanes per node split the stencil slots)"}
+                "oct": "k_stage_split32 (8 lanes per node split the stencil slots)",
+                "tile": "k_stage_tile32 (one lane per node over LDS tiles of {tn} nodes)"}
 
 
 def fp32_leg(pkg, torch, nodes, steps, warmup):
@@ -342,9 +343,10 @@ def fp32_leg(pkg, torch, nodes, steps, warmup):
     import numpy as np
     prob = pkg.host.Problem(pkg.mesh.case_path(nodes, seed=1), use_rcm=True, layout="cfdl",
                             fp32=True)
-    with pkg.runtime.HipContext(prob, fp32=True, kernel="lane") as base:
-        opts, _ = base.tune(members=1, steps=20)   # lanes per node: 1, 2, 4 or 8
-        with pkg.runtime.HipContext(prob, fp32=True, share=base, kernel=opts["kernel"]) as ctx:
+    with pkg.runtime.HipContext(prob, fp32=True, kernel="lane", order="hilbert") as base:
+        opts, _ = base.tune(members=1, steps=20)   # lanes per node 1, 2, 4, 8 or LDS tiles
+        with pkg.runtime.HipContext(prob, fp32=True, share=base, kernel=opts["kernel"],
+                                    tile_nodes=opts["tile_nodes"]) as ctx:
             ctx.set_state(prob.H_init)
             ms = timed_steps(torch, ctx, steps, warm=warmup)
             finite = bool(np.isfinite(ctx.get_state()).all())
@@ -352,7 +354,7 @@ def fp32_leg(pkg, torch, nodes, steps, warmup):
     achieved = stage_bytes(prob.padded, 1, fp32=True) / (launch_us * 1e-6) / 1e9
     return {"workload": f"N={prob.n_nodes}, 1 member, {steps} RK4 steps, FP32 "
                         "(the reference's RUN_32BIT type)",
-            "kernel": FP32_KERNELS[opts["kernel"]], "dtype": "f32",
+            "kernel": FP32_KERNELS[opts["kernel"]].format(tn=opts["tile_nodes"]), "dtype": "f32",
             "value": prob.n_nodes * 4 / (ms * 1e-3) / 1e9, "unit": "GNode-RHS/s",
             "ms_per_step": ms,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

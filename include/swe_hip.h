@@ -138,12 +138,15 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
                                    HILBERT node order for compact tiles)     */
 #define SWE_HIP_KERNEL_OCT 6    /* FP32 single state: 8 lanes per node
                                    splitting the stencil slots (fast mode)   */
-/* In SWE_HIP_FP32 contexts: LANE = one lane per node (the only mapping of
- * exact mode); PAIR, QUAD and OCT = 2, 4 or 8 lanes per node, each summing
- * every 2nd, 4th or 8th stencil slot, the partial sums added across the
- * group (fast mode); AUTO = LANE in exact mode, and in fast mode PAIR up to
- * 262144 nodes (one lane per node would leave the chip under-filled), LANE
- * above.  QUADW and TILE are FP64 mappings. */
+/* In SWE_HIP_FP32 contexts: LANE = one lane per node; TILE = one lane per
+ * node over LDS tiles of tile_nodes (64, 128 or 256) consecutive device
+ * nodes (31-point stencils; use a HILBERT node order); both sum every
+ * stencil in order (exact and fast mode).  PAIR, QUAD and OCT = 2, 4 or 8
+ * lanes per node, each summing every 2nd, 4th or 8th stencil slot, the
+ * partial sums added across the group (fast mode only).  AUTO = LANE in
+ * exact mode, and in fast mode PAIR up to 262144 nodes (one lane per node
+ * would leave the chip under-filled), LANE above.  QUADW is an FP64
+ * mapping. */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 #define SWE_HIP_ORDER_HILBERT 2 /* device node order = 3-D Hilbert curve     */
@@ -208,7 +211,8 @@ int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int st
  *     block 32 the member-lane tile shapes (tile_nodes 8 or 6 x tile_rows 70
  *     or 56);
  *   SWE_HIP_FP32 contexts (flags must carry SWE_HIP_FP32, members == 1):
- *     LANE, and in fast mode PAIR, QUAD and OCT (lanes per node).
+ *     LANE, TILE with 64, 128 and 256 nodes, and in fast mode PAIR, QUAD
+ *     and OCT (lanes per node).
  * No counterpart in the reference. */
 int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
                     swe_hip_options *best, double *best_ms_per_step);

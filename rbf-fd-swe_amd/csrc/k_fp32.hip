@@ -15,10 +15,27 @@ void launch_split32(const swe_hip_ctx *c, const DevMesh32 &m, DevStage32 st)
         k_stage_split32<STAGE, 0, S><<<g, b, 0, c->stream>>>(m, st);
 }
 
+template <int STAGE, bool EX, int TN>
+void launch_tile32_tn(const swe_hip_ctx *c, const DevMesh32 &m, DevStage32 st)
+{
+    const DevTiles tl = tiles_of(c, 1);
+    st.nblk = (tl.ntiles + 7) / 8 * 8;
+    k_stage_tile32<STAGE, EX, 31, TN><<<dim3(st.nblk), dim3(TN), 0, c->stream>>>(m, tl, st);
+}
+
 template <int STAGE, bool EX>
 void launch32(const swe_hip_ctx *c, DevStage32 st)
 {
     const DevMesh32 m = mesh32_of(c);
+    if (c->kernel == SWE_HIP_KERNEL_TILE && c->tiles.ntiles > 0 && c->nnbr == 31) {
+        if (c->tile_nodes == 64)
+            launch_tile32_tn<STAGE, EX, 64>(c, m, st);
+        else if (c->tile_nodes == 128)
+            launch_tile32_tn<STAGE, EX, 128>(c, m, st);
+        else
+            launch_tile32_tn<STAGE, EX, 256>(c, m, st);
+        return;
+    }
     if constexpr (!EX) {   // fast mode: S lanes per node split the slots
         switch (c->split32) {
         case 2: launch_split32<STAGE, 2>(c, m, st); return;

@@ -780,12 +780,12 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
                                     "tiled)");
     if (o->share && o->share->fp32 != fp32)
         return fail(SWE_HIP_EINVAL, "share: the shared context has the other model precision");
-    if (fp32 && (o->kernel == SWE_HIP_KERNEL_QUADW || o->kernel == SWE_HIP_KERNEL_TILE))
-        return fail(SWE_HIP_EINVAL, "SWE_HIP_FP32 kernels are AUTO, LANE, PAIR, QUAD and OCT");
+    if (fp32 && o->kernel == SWE_HIP_KERNEL_QUADW)
+        return fail(SWE_HIP_EINVAL, "SWE_HIP_FP32 kernels are AUTO, LANE, PAIR, QUAD, OCT and TILE");
     if (fp32 && (o->flags & SWE_HIP_EXACT) && o->kernel != SWE_HIP_KERNEL_AUTO
-        && o->kernel != SWE_HIP_KERNEL_LANE)
+        && o->kernel != SWE_HIP_KERNEL_LANE && o->kernel != SWE_HIP_KERNEL_TILE)
         return fail(SWE_HIP_EINVAL, "exact SWE_HIP_FP32 contexts sum each stencil in order: "
-                                    "kernel AUTO or LANE");
+                                    "kernel AUTO, LANE or TILE");
     if (!fp32 && o->kernel == SWE_HIP_KERNEL_OCT)
         return fail(SWE_HIP_EINVAL, "SWE_HIP_KERNEL_OCT is an FP32 mapping");
     if (o->members == 1 && o->tile_nodes != 0 && o->tile_nodes != 64 && o->tile_nodes != 128
@@ -901,7 +901,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         k_set_int<<<1, 64, 0, c->stream>>>(c->d_active, c->B);
         rc = check_launch("k_set_int");
     }
-    if (!rc && c->B == 1 && c->kernel == SWE_HIP_KERNEL_TILE && !fp32) {
+    if (!rc && c->B == 1 && c->kernel == SWE_HIP_KERNEL_TILE && (!fp32 || c->nnbr == 31)) {
         if (!c->tile_nodes)
             c->tile_nodes = kPtileNodes;
         rc = build_tiles(c, c->tile_nodes, 4 * c->tile_nodes);
@@ -1020,11 +1020,14 @@ int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
         cand.push_back(swe_hip_options{members, flags, base->dev, mb, base, kernel, prefetch, 0,
                                        tiled, 0, tn, tr});
     };
-    if (base->fp32) {   // lanes per node; exact mode sums every stencil in one lane
+    if (base->fp32) {   // lanes per node (exact mode: one, in order) and LDS tiles
         add(0, SWE_HIP_KERNEL_LANE, 0, 0, 0, 0);
         if (!(flags & SWE_HIP_EXACT))
             for (int k : {SWE_HIP_KERNEL_PAIR, SWE_HIP_KERNEL_QUAD, SWE_HIP_KERNEL_OCT})
                 add(0, k, 0, 0, 0, 0);
+        if (base->nnbr == 31)
+            for (int tn : {64, 128, 256})
+                add(0, SWE_HIP_KERNEL_TILE, 0, 0, tn, 0);
     } else if (members == 1) {
         add(0, SWE_HIP_KERNEL_PAIR, 0, 0, 0, 0);
         if (base->nnbr == 31)

@@ -276,4 +276,92 @@ __global__ __launch_bounds__(256) void k_stage_split32(DevMesh32 m, DevStage32 s
     stage_update32<STAGE, false>(st, 4L * node, F);
 }
 
+// Single state over LDS tiles (k_stage_ptile's scheme in FP32, one lane
+// per node): a workgroup of TN lanes takes a tile of <= TN consecutive
+// (Hilbert-ordered) nodes, copies the tile's <= 4 TN distinct neighbour
+// states (16-byte rows, one direct-to-LDS load per row) into LDS, then sums
+// every stencil slot in order from LDS through the 16-bit tile positions
+// (node-major, 64 B per node) -- the same sums in the same order as
+// k_stage_lane32 (bitwise in exact mode), with the 31 scattered gathers and
+// 31 ids per node replaced by LDS reads and 64 B of positions.
+template <int STAGE, bool EXACT, int NNBR, int TN>
+__global__ __launch_bounds__(TN) void k_stage_tile32(DevMesh32 m, DevTiles tl, DevStage32 st)
+{
+    static_assert(NNBR > 0 && NNBR <= 31, "positions: 31 slots + the own row");
+    static_assert(TN % 64 == 0, "whole waves");
+    __shared__ float4 rows[4 * TN];
+    const int tile = xcd_block(st.nblk);
+    if (tile >= tl.ntiles)
+        return;   // uniform over the workgroup
+    const int t0 = tl.tile_start[tile];
+    const int tcount = tl.tile_start[tile + 1] - t0;
+    const int u0 = tl.ustart[tile];
+    const int ucount = tl.ustart[tile + 1] - u0;
+    const float *__restrict__ src = st.src;
+    const int lane = (int)threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    // wave w copies rows [64 c, 64 c + 64) for c = w, w + waves, ...
+    for (int c = wv; 64 * c < ucount; c += TN / 64) {
+        const int r = 64 * c + lane;
+        const int u = tl.ulist[u0 + (r < ucount ? r : ucount - 1)];
+        if (r < ucount)
+            dma16(src + 4L * u, reinterpret_cast<double *>(rows + 64 * c));
+    }
+    const int nl = (int)threadIdx.x;
+    const bool live = nl < tcount;
+    const int node = t0 + (live ? nl : tcount - 1);
+    const long ld = m.sblk ? 16L : (long)m.ld;
+    const long sb = m.sblk ? ((long)(node >> 4) * NNBR << 4) + (node & 15) : (long)node;
+    const float *__restrict__ wp = m.wts + 4L * sb;
+    const uint4 *pp = reinterpret_cast<const uint4 *>(tl.lidx_node + 32L * node);
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's row loads are in LDS
+    __syncthreads();
+    Sums32 S = {};
+    // groups of 8 slots (one 16-byte load of their positions), rolled so
+    // that ~8 slots of weight loads are in flight per lane: unrolled over
+    // all 31 slots the weight loads are hoisted and take 266 registers
+    unsigned ownw = 0xffffu;
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+        const uint4 pv = pp[g];
+        const unsigned pw[4] = {pv.x, pv.y, pv.z, pv.w};
+        if (g == 3)
+            ownw = pv.w >> 16;   // spare slot 31: the node's own row
+        static_for<8>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const int j = 8 * g + k;
+            if (NNBR % 8 == 0 || j < NNBR) {
+                const int pos = (int)((k & 1) ? (pw[k >> 1] >> 16) : (pw[k >> 1] & 0xffffu));
+                const float4 w = *reinterpret_cast<const float4 *>(wp + 4L * j * ld);
+                const float4 v = rows[pos];
+                S.dudx = madd32<EXACT>(S.dudx, w.x, v.x); S.dvdx = madd32<EXACT>(S.dvdx, w.x, v.y);
+                S.dwdx = madd32<EXACT>(S.dwdx, w.x, v.z); S.dhdx = madd32<EXACT>(S.dhdx, w.x, v.w);
+                S.dudy = madd32<EXACT>(S.dudy, w.y, v.x); S.dvdy = madd32<EXACT>(S.dvdy, w.y, v.y);
+                S.dwdy = madd32<EXACT>(S.dwdy, w.y, v.z); S.dhdy = madd32<EXACT>(S.dhdy, w.y, v.w);
+                S.dudz = madd32<EXACT>(S.dudz, w.z, v.x); S.dvdz = madd32<EXACT>(S.dvdz, w.z, v.y);
+                S.dwdz = madd32<EXACT>(S.dwdz, w.z, v.z); S.dhdz = madd32<EXACT>(S.dhdz, w.z, v.w);
+                S.lu = madd32<EXACT>(S.lu, w.w, v.x); S.lv = madd32<EXACT>(S.lv, w.w, v.y);
+                S.lw = madd32<EXACT>(S.lw, w.w, v.z); S.lh = madd32<EXACT>(S.lh, w.w, v.w);
+            }
+        });
+    }
+    // own state: the node's row, else a global load (nodes outside their
+    // own stencil)
+    float4 own;
+    if (ownw != 0xffffu) {
+        own = rows[ownw];
+    } else {
+        own = *reinterpret_cast<const float4 *>(src + 4L * node);
+        asm volatile("" : "+v"(own.x), "+v"(own.y), "+v"(own.z), "+v"(own.w));
+    }
+    float g[GEO_N];
+#pragma unroll
+    for (int f = 0; f < GEO_N; ++f)
+        g[f] = m.geo[(long)f * m.ld + node];
+    float F[4];
+    rhs_point32(S, own.x, own.y, own.z, own.w, g, m.gh0, F);
+    if (live)
+        stage_update32<STAGE, EXACT>(st, 4L * node, F);
+}
+
 }  // namespace swe

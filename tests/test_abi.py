@@ -88,14 +88,16 @@ def test_bad_options_rejected_before_device(pkg):
 
 
 def test_fp32_kernel_options_rejected_before_device(pkg):
-    """Exact FP32 contexts sum every stencil in one lane; OCT is FP32-only;
-    QUADW and TILE are FP64 mappings."""
+    """Exact FP32 contexts sum every stencil in order (one lane per node, or
+    LDS tiles); OCT is FP32-only; QUADW is an FP64 mapping; tile sizes are
+    64, 128 or 256."""
     path = pkg.mesh.case_path(642, seed=1)
     p32 = pkg.host.Problem(path, use_rcm=True, layout="cfdl", fp32=True)
     p64 = pkg.host.Problem(path, use_rcm=True, layout="cfdl")
     bad = [(p32, dict(fp32=True, exact=True, kernel="pair")),
            (p32, dict(fp32=True, exact=True, kernel="quad")),
-           (p32, dict(fp32=True, kernel="quadw")), (p32, dict(fp32=True, kernel="tile")),
+           (p32, dict(fp32=True, kernel="quadw")), (p32, dict(fp32=True, kernel="tile",
+                                                            tile_nodes=100)),
            (p64, dict(kernel="oct"))]
     for prob, kw in bad:
         with pytest.raises(pkg.runtime.SweHipError) as ei:

@@ -104,7 +104,7 @@ def test_fp32_tuner_and_split_kernels_agree(pkg):
     print("case: FP32 tuner N=10242", flush=True)
     with pkg.runtime.HipContext(prob, fp32=True, kernel="lane") as base:
         opts, ms = base.tune(members=1, steps=3)
-        assert opts["kernel"] in ("lane", "pair", "quad", "oct") and ms > 0
+        assert opts["kernel"] in ("lane", "pair", "quad", "oct", "tile") and ms > 0
         out = {}
         for k in ("lane", "pair", "quad", "oct"):
             with pkg.runtime.HipContext(prob, fp32=True, share=base, kernel=k) as ctx:
@@ -114,3 +114,27 @@ def test_fp32_tuner_and_split_kernels_agree(pkg):
     for k in ("pair", "quad", "oct"):
         err = rel_by_var(out[k], out["lane"], prob)
         assert (err <= 1e-4).all(), (k, err)
+
+
+@pytest.mark.parametrize("tile_nodes", [64, 128, 256])
+def test_fp32_tile_kernel_matches_lane_kernel_bitwise(pkg, pyoracle, tile_nodes):
+    """The FP32 LDS-tile kernel sums every stencil in the lane kernel's order:
+    exact mode bitwise equal to the strict FP32 oracle, fast mode bitwise
+    equal to the one-lane fast kernel (Hilbert-ordered device copy)."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl",
+                            fp32=True)
+    print(f"case: FP32 tile kernel N=10242 tile_nodes={tile_nodes}", flush=True)
+    want = pyoracle.Oracle(prob, "strict32").run(prob.H_init, 5)
+    with pkg.runtime.HipContext(prob, fp32=True, exact=True, kernel="tile", order="hilbert",
+                                tile_nodes=tile_nodes) as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(5)
+        assert np.array_equal(ctx.get_state(), want)
+    out = {}
+    for k in ("lane", "tile"):
+        with pkg.runtime.HipContext(prob, fp32=True, kernel=k, order="hilbert",
+                                    tile_nodes=tile_nodes if k == "tile" else 0) as ctx:
+            ctx.set_state(prob.H_init)
+            ctx.rk_steps(20)
+            out[k] = ctx.get_state()
+    assert np.array_equal(out["tile"], out["lane"])
