@@ -457,6 +457,17 @@ def main():
     serving = None
     if not args.no_serve:
         serving = serve_leg(pkg, args.serve_requests, args.serve_rate, args.serve_kernel)
+    # the best published figure for this workload: the reference's own run log
+    # (Eval_Rhs 7.371213e-04 s per step, N = 40962, sfdl layout, OpenACC on a
+    # V100, 2 MPI ranks), which times the RHS alone (value also includes the
+    # fused RK4 updates)
+    vs_baseline, vs_basis = None, None
+    if prob.n_nodes == 40962 and args.members == 1 and not args.fp32:
+        published = 4 * 40962 / 7.371213e-04 / 1e9
+        vs_baseline = value / published
+        vs_basis = (f"{published:.3f} GNode-RHS/s: Eval_Rhs of the reference's run log "
+                    "run/psg/output/output_sfdl.txt:41 (N=40962, sfdl, OpenACC on a V100, "
+                    "2 MPI ranks, precision not recorded); it times the RHS alone")
     out = {
         "metric": "eval_RHS GNodes/s + HBM GB/s; ensemble steps/s and p50/p99 latency",
         "value": value,
@@ -467,7 +478,8 @@ def main():
         "ms_per_step": gpu_ms / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": vs_baseline,
+        "vs_baseline_basis": vs_basis,
         "dtype": "f32" if args.fp32 else "f64",
         "data": "synthetic (seeded icosahedral mesh + Gaussian RBF-FD weights, TC5-like state)",
         "config": {"workload": f"RBF-FD SWE RK4, N={prob.n_nodes} nodes, stencil 31, "
