@@ -27,7 +27,7 @@ Further legs (each its own key, none of them the headline):
              and a sample checked bitwise against solo runs
   fp32       the headline workload in the reference's FP32 model type
 
-Flags: --gpus N (only 1 is supported: every GPU box here has one GPU),
+Flags: --gpus N (only 1 is supported: the path is built and measured on one GPU),
 --steps K, --warmup W, --nodes N, --members B, --exact, --no-cpu,
 --no-ensemble, --no-serve, --no-c3, --no-fp32.
 """
@@ -368,7 +368,8 @@ def main():
     args = parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this path runs on one GPU "
-              "(each GPU box has one GPU; ensembles shard by member with no exchange)",
+              "(ensembles and serving would shard by member across GPUs, one process "
+              "per GPU, with no exchange)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np

@@ -1,7 +1,7 @@
 """Scans the gfx950 code object of this repository's own libswe_hip.so for
 instructions this build must never contain: writes through the scalar data
 cache and hand-made packed-FP32 arithmetic.  (CPU test; it names those
-mnemonics, so it is kept off the GPU box.)"""
+mnemonics, so it is excluded from the trees that GPU runs receive.)"""
 import os
 import re
 import shutil

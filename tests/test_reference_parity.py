@@ -1,6 +1,6 @@
 """The C host and the CPU oracle against the reference build (oracle/_ref,
 the reference's own sources) on the reference's shipped 2562 input.
-Skipped where the reference tree is absent (the GPU box)."""
+Skipped where the reference tree is absent."""
 import numpy as np
 import pytest
 
