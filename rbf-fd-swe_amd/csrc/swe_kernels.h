@@ -1553,4 +1553,5 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
     }
 }
 
+
 }  // namespace swe
