@@ -1553,5 +1553,4 @@ __global__ __launch_bounds__(512) void k_stage_tspec(DevMesh m, DevTiles tl, Dev
     }
 }
 
-
 }  // namespace swe

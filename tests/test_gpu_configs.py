@@ -172,4 +172,3 @@ def test_c4_matrix_core_sums_vs_oracle_and_composition(pkg, pyoracle, c4):
         err = rel_by_var(got[m], orc.run(inits[m], 10), prob)
         print(f"member {m} rel err per var {err}", flush=True)
         assert (err <= 1e-12).all(), (m, err)
-
