@@ -157,11 +157,10 @@ def dominant_kernel(args):
     if args.members == 1:
         if args.kernel == "auto":
             return "k_stage_pair"
-        return {"quadw": "k_stage_quadw", "quad": "k_stage_quad",
-                "lane": "k_stage_lane", "pair": "k_stage_pair",
+        return {"quadw": "k_stage_quadw", "pair": "k_stage_pair",
                 "tile": "k_stage_ptile"}[args.kernel]
     if args.tiled:
-        return "k_stage_tspec" if args.member_block == 32 else "k_stage_tiled"
+        return "k_stage_tspec"
     return "k_stage_mpair" if args.member_block in (0, 64) and args.members % 64 == 0 \
         else "k_stage_members"
 
@@ -190,11 +189,9 @@ def describe(opts):
             return (f"k_stage_ptile (pair mapping over LDS tiles of {opts['tile_nodes']} nodes, "
                     f"row staging {opts['prefetch']})")
         return "k_stage_pair"
-    if opts["tiled"] and opts["member_block"] == 32:
+    if opts["tiled"]:
         return (f"k_stage_tspec (member-lane LDS tiles, {opts['tile_nodes'] or 8} nodes x "
                 f"{opts['tile_rows'] or 70} rows, loader and compute waves, direct-to-LDS loads)")
-    if opts["tiled"]:
-        return f"k_stage_tiled (member block {opts['member_block']})"
     if opts["member_block"] == 64:
         return "k_stage_mpair (member pairs, 16-byte gathers)"
     return f"k_stage_members (member block {opts['member_block']})"

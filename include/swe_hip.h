@@ -123,8 +123,9 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
  * options fail with SWE_HIP_EINVAL before any device call. */
 #define SWE_HIP_KERNEL_AUTO 0   /* single state: PAIR with a 31-point
                                    stencil, else QUADW                       */
-#define SWE_HIP_KERNEL_QUAD 1   /* single state: 4 lanes per node            */
-#define SWE_HIP_KERNEL_LANE 2   /* single state: 1 lane per node             */
+#define SWE_HIP_KERNEL_QUAD 1   /* FP32 single state: 4 lanes per node
+                                   splitting the stencil slots (fast mode)   */
+#define SWE_HIP_KERNEL_LANE 2   /* FP32 single state: 1 lane per node        */
 #define SWE_HIP_KERNEL_QUADW 3  /* single state: 4 lanes per node, weights,
                                    ids, geometry loaded once per quad and
                                    shared by DPP                             */
@@ -166,11 +167,10 @@ typedef struct swe_hip_options {
     int node_order;             /* SWE_HIP_ORDER_* of the device copy (ignored
                                    with share: the mesh's order is used)    */
     int tiled;                  /* 1: LDS-staged ensemble kernel (members
-                                   >= 2, member_block 4, 8 or 16: one member
-                                   per lane; member_block 32 with a 31-point
-                                   stencil: persistent member-lane tiles,
+                                   >= 32, member_block 32, 31-point
+                                   stencil): persistent member-lane tiles,
                                    loader and compute waves over two LDS
-                                   images, 160 KB per workgroup)            */
+                                   images, 160 KB per workgroup             */
     int slot_block;             /* stencil arrays: 0 slot-major [nnbr][n];
                                    16 node blocks [n/16][nnbr][16] (ignored
                                    with share)                              */
@@ -191,9 +191,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
 int swe_hip_tune_member_block(swe_hip_ctx *base, int members, unsigned flags, int steps,
                               int *best_mb, double *best_ms_per_step);
 
-/* The same over the LDS-tiled ensemble kernels too: member blocks 4, 8, 16
- * (one member per lane) and 32 (member-lane tiles, 31-point stencils) with
- * tiled = 1.  Returns the fastest (member block, tiled) pair; tiles follow
+/* The same over the LDS-tiled ensemble kernel too (member block 32,
+ * member-lane tiles, 31-point stencils, tiled = 1).  Returns the fastest (member block, tiled) pair; tiles follow
  * the node order of base's device copy (create base with
  * SWE_HIP_ORDER_HILBERT for compact tiles).  No counterpart in the
  * reference (its ensemble runs are separate processes). */
@@ -207,9 +206,8 @@ int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int st
  *     and 256 nodes and each row-staging schedule (prefetch 0: through
  *     registers, 4: direct-to-LDS, 8: direct-to-LDS with the first weights
  *     loaded before the tile barrier);
- *   members > 1: every member block, untiled and LDS-tiled, and for member
- *     block 32 the member-lane tile shapes (tile_nodes 8 or 6 x tile_rows 70
- *     or 56);
+ *   members > 1: every member block untiled, and member block 32 on
+ *     member-lane tiles of tile_nodes 8 or 6 x tile_rows 70 or 56;
  *   SWE_HIP_FP32 contexts (flags must carry SWE_HIP_FP32, members == 1):
  *     LANE, TILE with 64, 128 and 256 nodes, and in fast mode PAIR, QUAD
  *     and OCT (lanes per node).

@@ -1,34 +1,8 @@
-// Launchers of the single-state stage kernels (quad, quadw, pair, lane mappings).
+// Launchers of the FP64 single-state stage kernels (pair, tile and quadw mappings).
 #include "swe_ctx.h"
 
 namespace swe {
 namespace {
-
-template <int STAGE, bool EX, int NN>
-void launch_quad(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
-{
-    st.nblk = blocks_x(c, 4L * c->n, kSingleBlock);
-    const dim3 g(st.nblk), b(kSingleBlock);
-    if (NN > 0 && c->prefetch == 8)
-        k_stage_quad<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
-    else if (NN > 0 && c->prefetch == 4)
-        k_stage_quad<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
-    else
-        k_stage_quad<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
-}
-
-template <int STAGE, bool EX, int NN>
-void launch_lane(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
-{
-    st.nblk = blocks_x(c, (long)c->n, kSingleBlock);
-    const dim3 g(st.nblk), b(kSingleBlock);
-    if (NN > 0 && (EX || c->prefetch == 4))
-        k_stage_lane<STAGE, EX, NN, 4><<<g, b, 0, c->stream>>>(m, st);
-    else if (NN > 0 && c->prefetch == 8)
-        k_stage_lane<STAGE, EX, NN, 8><<<g, b, 0, c->stream>>>(m, st);
-    else
-        k_stage_lane<STAGE, EX, NN, 0><<<g, b, 0, c->stream>>>(m, st);
-}
 
 template <int STAGE, bool EX, int NN>
 void launch_quadw(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
@@ -92,16 +66,12 @@ template <int STAGE, bool EX, int NN>
 void family_single(const swe_hip_ctx *c, const DevStage &st)
 {
     const DevMesh m = mesh_of(c);
-    if (c->kernel == SWE_HIP_KERNEL_LANE)
-        launch_lane<STAGE, EX, NN>(c, m, st);
-    else if (c->kernel == SWE_HIP_KERNEL_QUADW)
-        launch_quadw<STAGE, EX, NN>(c, m, st);
-    else if (c->kernel == SWE_HIP_KERNEL_PAIR)
+    if (c->kernel == SWE_HIP_KERNEL_PAIR)
         launch_pair<STAGE, EX, NN>(c, m, st);
     else if (c->kernel == SWE_HIP_KERNEL_TILE && c->tiles.ntiles > 0)
         launch_ptile<STAGE, EX, NN>(c, m, st);
     else
-        launch_quad<STAGE, EX, NN>(c, m, st);
+        launch_quadw<STAGE, EX, NN>(c, m, st);
 }
 
 namespace {

@@ -1,17 +1,9 @@
-// Launchers of the LDS-staged (tiled) ensemble stage kernel.
+// Launcher of the LDS-staged (tiled) ensemble stage kernel: member lanes over
+// LDS tiles.
 #include "swe_ctx.h"
 
 namespace swe {
 namespace {
-
-template <int STAGE, bool EX, int MBv, int NN>
-void launch_tiled(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
-{
-    const DevTiles tl = tiles_of(c, MBv);
-    st.nblk = (int)(((long)tl.ntiles * tl.nmb + 7) / 8 * 8);
-    const size_t lds = sizeof(double) * 4 * MBv * (size_t)c->tiles.umax;
-    k_stage_tiled<STAGE, EX, MBv, NN><<<dim3(st.nblk), dim3(kTiledBlock), lds, c->stream>>>(m, tl, st);
-}
 
 // Member lanes over LDS tiles (member block 32, 31-point stencils): loader
 // and compute waves over two LDS images, one persistent workgroup of 8 waves
@@ -31,40 +23,16 @@ void launch_tspec(const swe_hip_ctx *c, const DevMesh &m, DevStage st)
         k_stage_tspec<STAGE, EX, 31><<<dim3(st.nblk), dim3(512), lds, c->stream>>>(m, tl, st);
 }
 
-}  // namespace
-
-template <int STAGE, bool EX, int NN>
-void family_tiled(const swe_hip_ctx *c, const DevStage &st)
-{
-    const DevMesh m = mesh_of(c);
-    switch (c->MB) {
-    case 4: launch_tiled<STAGE, EX, 4, NN>(c, m, st); break;
-    case 8: launch_tiled<STAGE, EX, 8, NN>(c, m, st); break;
-    default: launch_tiled<STAGE, EX, 16, NN>(c, m, st); break;
-    }
-}
-
-namespace {
-
 template <int STAGE>
 void by_flags(const swe_hip_ctx *c, const DevStage &st)
 {
-    if (c->MB == 32) {   // set up only for 31-point stencils
-        const DevMesh m = mesh_of(c);
-        if (exact(c))
-            launch_tspec<STAGE, true>(c, m, st);
-        else
-            launch_tspec<STAGE, false>(c, m, st);
-        return;
-    }
-    const bool nn31 = c->nnbr == 31;
-    if (exact(c)) {
-        if (nn31) family_tiled<STAGE, true, 31>(c, st);
-        else family_tiled<STAGE, true, 0>(c, st);
-    } else {
-        if (nn31) family_tiled<STAGE, false, 31>(c, st);
-        else family_tiled<STAGE, false, 0>(c, st);
-    }
+    // tiled contexts have member block 32 and a 31-point stencil (checked at
+    // creation)
+    const DevMesh m = mesh_of(c);
+    if (exact(c))
+        launch_tspec<STAGE, true>(c, m, st);
+    else
+        launch_tspec<STAGE, false>(c, m, st);
 }
 
 }  // namespace

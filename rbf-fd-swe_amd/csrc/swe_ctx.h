@@ -19,8 +19,6 @@ constexpr int kBlock = 256;            // threads per workgroup, stage kernels
 // (at N = 40962 the pair kernel has 1281 waves: 256-thread workgroups put
 // two on 65 CUs and one on the rest).
 constexpr int kSingleBlock = 64;
-constexpr int kTiledBlock = 512;       // threads per workgroup, tiled kernel
-constexpr int kTiledLds = 80 * 1024;   // LDS per tiled workgroup: two per CU
 constexpr int kGraphCount = 4;         // captured step graphs per context
 
 // Host index macros of the reference (swe_config.h:21-54) at run time.

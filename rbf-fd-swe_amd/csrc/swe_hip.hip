@@ -715,7 +715,7 @@ int tune_candidates(swe_hip_ctx *base, int members, unsigned flags, int steps, b
         for (int mb = 64; mb >= 2; mb >>= 1) {
             if (members % mb)
                 continue;
-            if (tiled && !((mb >= 4 && mb <= 16) || (mb == 32 && base->nnbr == 31)))
+            if (tiled && !(mb == 32 && members >= 32 && base->nnbr == 31))
                 continue;
             double per = 0.0;
             const swe_hip_options o{members, flags, base->dev, mb, base, 0, 0, 0, tiled, 0, 0, 0};
@@ -786,8 +786,10 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         && o->kernel != SWE_HIP_KERNEL_LANE && o->kernel != SWE_HIP_KERNEL_TILE)
         return fail(SWE_HIP_EINVAL, "exact SWE_HIP_FP32 contexts sum each stencil in order: "
                                     "kernel AUTO, LANE or TILE");
-    if (!fp32 && o->kernel == SWE_HIP_KERNEL_OCT)
-        return fail(SWE_HIP_EINVAL, "SWE_HIP_KERNEL_OCT is an FP32 mapping");
+    if (!fp32 && (o->kernel == SWE_HIP_KERNEL_OCT || o->kernel == SWE_HIP_KERNEL_QUAD
+                  || o->kernel == SWE_HIP_KERNEL_LANE))
+        return fail(SWE_HIP_EINVAL, "FP64 single-state kernels are AUTO, PAIR, QUADW and TILE "
+                                    "(QUAD, LANE and OCT are FP32 mappings)");
     if (o->members == 1 && o->tile_nodes != 0 && o->tile_nodes != 64 && o->tile_nodes != 128
         && o->tile_nodes != 256)
         return fail(SWE_HIP_EINVAL, "tile_nodes of the single-state tile kernel must be 0, 64, "
@@ -805,6 +807,9 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     if (mb < 1)
         return fail(SWE_HIP_EINVAL, "member_block must be a power of two in 2..64 dividing "
                                     "members (members > 1 must be even)");
+    if (o->tiled && (mb != 32 || o->members < 32))
+        return fail(SWE_HIP_EINVAL, "the tiled ensemble kernel takes member block 32 (member "
+                                    "lanes over LDS tiles)");
     int device = o->device;
     if (o->share)
         device = o->share->dev;
@@ -908,7 +913,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     }
     if (!rc && o->tiled) {
         int lds_max = 0;
-        if (c->MB == 32 && c->B >= 32) {
+        {
             if (c->nnbr != 31)
                 rc = fail(SWE_HIP_EINVAL, "the member-lane tiled kernel (member block 32) needs "
                                           "a 31-point stencil");
@@ -920,11 +925,6 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
             else
                 rc = build_tiles(c, c->tile_nodes ? c->tile_nodes : kTmemTile,
                                  c->tile_rows ? c->tile_rows : kTmemRows);
-        } else if (c->B < 2 || (c->MB != 4 && c->MB != 8 && c->MB != 16)) {
-            rc = fail(SWE_HIP_EINVAL, "the tiled ensemble kernel needs members >= 2 and a "
-                                      "member block of 4, 8, 16 or 32");
-        } else {
-            rc = build_tiles(c, kTiledBlock / c->MB, kTiledLds / (32 * c->MB));
         }
     }
     if (!rc)
@@ -1039,8 +1039,6 @@ int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
             if (members % mb)
                 continue;
             add(mb, 0, 0, 0, 0, 0);
-            if (mb >= 4 && mb <= 16)
-                add(mb, 0, 0, 1, 0, 0);
             if (mb == 32 && base->nnbr == 31)   // member-lane tiles: nodes x rows per tile
                 for (int tn : {8, 6})
                     for (int tr : {kTmemRows, 56})

@@ -311,80 +311,21 @@ __device__ __forceinline__ void stencil_loop(const int *__restrict__ nb, long ld
     }
 }
 
-struct QuadSlot {
-    double2 w01, w23;
-    double s;
-};
-
-// ---------------------------------------------------------------------------
-// Single state (B == 1), quad mapping: four lanes per node.  Lane k of the
-// quad reads variable k of every neighbour and keeps the Dx, Dy, Dz, L sums
-// of that variable (each a sequential sum over the stencil); the 16 sums are
-// then shared across the quad by DPP, every lane evaluates the pointwise RHS
-// and lane k updates variable k.  256 threads = 64 nodes per workgroup.
-template <int STAGE, bool EXACT, int NNBR, int PF>
-__global__ __launch_bounds__(256) void k_stage_quad(DevMesh m, DevStage st)
-{
-    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
-    const int node_raw = t >> 2;
-    const int k = t & 3;
-    const bool live = node_raw < m.n;
-    const int node = live ? node_raw : m.n - 1;  // keep quads whole for DPP
-    const int nn = NNBR > 0 ? NNBR : m.nnbr;
-    const long ld = slot_stride(m);
-    const long sb = slot_base<NNBR>(m, node);
-    const int *__restrict__ nb = m.nbr + sb;
-    const double *__restrict__ wp = m.wts + 4L * sb;
-    const double *__restrict__ src = st.src;
-
-    double sx = 0.0, sy = 0.0, sz = 0.0, sl = 0.0;
-    auto load = [&](int j, int q) {
-        QuadSlot r;
-        r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
-        r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
-        r.s = src[4L * q + k];
-        return r;
-    };
-    auto acc = [&](const QuadSlot &r) {
-        sx = madd<EXACT>(sx, r.w01.x, r.s);
-        sy = madd<EXACT>(sy, r.w01.y, r.s);
-        sz = madd<EXACT>(sz, r.w23.x, r.s);
-        sl = madd<EXACT>(sl, r.w23.y, r.s);
-    };
-    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (QuadSlot *)nullptr);
-    Sums S;
-    S.dudx = quad_bcast<0>(sx); S.dudy = quad_bcast<0>(sy);
-    S.dudz = quad_bcast<0>(sz); S.lu = quad_bcast<0>(sl);
-    S.dvdx = quad_bcast<1>(sx); S.dvdy = quad_bcast<1>(sy);
-    S.dvdz = quad_bcast<1>(sz); S.lv = quad_bcast<1>(sl);
-    S.dwdx = quad_bcast<2>(sx); S.dwdy = quad_bcast<2>(sy);
-    S.dwdz = quad_bcast<2>(sz); S.lw = quad_bcast<2>(sl);
-    S.dhdx = quad_bcast<3>(sx); S.dhdy = quad_bcast<3>(sy);
-    S.dhdz = quad_bcast<3>(sz); S.lh = quad_bcast<3>(sl);
-
-    const double2 s01 = *reinterpret_cast<const double2 *>(src + 4L * node);
-    const double2 s23 = *reinterpret_cast<const double2 *>(src + 4L * node + 2);
-    const Geo g = load_geo(m.geo, m.ld, node);
-    double F[4];
-    rhs_point(S, s01.x, s01.y, s23.x, s23.y, g, m.gh0, F);
-    const double Fk = k == 0 ? F[0] : k == 1 ? F[1] : k == 2 ? F[2] : F[3];
-    if (live)
-        stage_update<STAGE, EXACT>(st, 4L * node + k, Fk);
-}
-
 struct QuadWSlot {
     double w;
     double s;
 };
 
-// Single state, quad mapping with shared loads: lane k of the quad loads
-// only weight k (Dx, Dy, Dz or L) of each slot -- one 8-byte load per lane,
-// a wave's 16 nodes x 32 B read once instead of four times -- and the quad
+// ---------------------------------------------------------------------------
+// Single state (B == 1), quad mapping with shared loads (any stencil width):
+// four lanes per node, lane k reading variable k of every neighbour and
+// keeping the Dx, Dy, Dz, L sums of that variable (each a sequential sum
+// over the stencil).  Lane k loads only weight k of each slot -- one 8-byte
+// load per lane, a wave's 16 nodes x 32 B read once -- and the quad
 // exchanges the four weights by DPP; the stencil ids, the node's own state
 // and its geometry are likewise read once per quad and shared by DPP.  A
 // wave64 memory instruction costs the CU's address unit the same time
-// whatever it fetches, so every load removed counts.  Same sums, same order
-// as k_stage_quad.
+// whatever it fetches, so every load removed counts.
 template <int STAGE, bool EXACT, int NNBR, int PF>
 __global__ __launch_bounds__(256) void k_stage_quadw(DevMesh m, DevStage st)
 {
@@ -560,60 +501,6 @@ __global__ __launch_bounds__(256) void k_stage_pair(DevMesh m, DevStage st)
     rhs_point(S, u, v, w, h, g, m.gh0, F);
     if (live)
         stage_update_pair<STAGE, EXACT>(st, 4L * node + 2 * k, k ? F[2] : F[0], k ? F[3] : F[1]);
-}
-
-struct LaneSlot {
-    double2 w01, w23, s01, s23;
-};
-
-// Single state (B == 1), lane mapping: one lane per node with all 16 sums.
-// A quarter of the quad kernel's load instructions per node (each lane
-// reads its own 32 B of weights and the neighbour's whole 32 B state), for
-// meshes large enough to fill the chip with one lane per node.
-template <int STAGE, bool EXACT, int NNBR, int PF>
-__global__ __launch_bounds__(256) void k_stage_lane(DevMesh m, DevStage st)
-{
-    const int t = xcd_block(st.nblk) * (int)blockDim.x + (int)threadIdx.x;
-    const bool live = t < m.n;
-    const int node = live ? t : m.n - 1;
-    const int nn = NNBR > 0 ? NNBR : m.nnbr;
-    const long ld = slot_stride(m);
-    const long sb = slot_base<NNBR>(m, node);
-    const int *__restrict__ nb = m.nbr + sb;
-    const double *__restrict__ wp = m.wts + 4L * sb;
-    const double *__restrict__ src = st.src;
-
-    Sums S = {};
-    auto load = [&](int j, int q) {
-        LaneSlot r;
-        r.w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
-        r.w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
-        r.s01 = *reinterpret_cast<const double2 *>(src + 4L * q);
-        r.s23 = *reinterpret_cast<const double2 *>(src + 4L * q + 2);
-        return r;
-    };
-    auto acc = [&](const LaneSlot &r) {
-        const double su = r.s01.x, sv = r.s01.y, sw = r.s23.x, sh = r.s23.y;
-        S.dudx = madd<EXACT>(S.dudx, r.w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, r.w01.x, sv);
-        S.dwdx = madd<EXACT>(S.dwdx, r.w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, r.w01.x, sh);
-        S.dudy = madd<EXACT>(S.dudy, r.w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, r.w01.y, sv);
-        S.dwdy = madd<EXACT>(S.dwdy, r.w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, r.w01.y, sh);
-        S.dudz = madd<EXACT>(S.dudz, r.w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, r.w23.x, sv);
-        S.dwdz = madd<EXACT>(S.dwdz, r.w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, r.w23.x, sh);
-        S.lu = madd<EXACT>(S.lu, r.w23.y, su); S.lv = madd<EXACT>(S.lv, r.w23.y, sv);
-        S.lw = madd<EXACT>(S.lw, r.w23.y, sw); S.lh = madd<EXACT>(S.lh, r.w23.y, sh);
-    };
-    stencil_loop<NNBR, PF>(nb, ld, nn, load, acc, (LaneSlot *)nullptr);
-    const double2 s01 = *reinterpret_cast<const double2 *>(src + 4L * node);
-    const double2 s23 = *reinterpret_cast<const double2 *>(src + 4L * node + 2);
-    const Geo g = load_geo(m.geo, m.ld, node);
-    double F[4];
-    rhs_point(S, s01.x, s01.y, s23.x, s23.y, g, m.gh0, F);
-    if (live) {
-#pragma unroll
-        for (int var = 0; var < 4; ++var)
-            stage_update<STAGE, EXACT>(st, 4L * node + var, F[var]);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -814,14 +701,12 @@ __global__ __launch_bounds__(256) void k_stage_mpair(DevMesh m, DevStage st)
 }
 
 // ---------------------------------------------------------------------------
-// Ensemble, tiled (B >= 2): the mesh is stored in a space-filling-curve node
-// order and cut into tiles of consecutive nodes whose stencils share few
-// neighbours (a 64-node tile touches ~220 distinct nodes, 9x reuse).  One
-// workgroup = one tile x MB members: it first copies the tile's distinct
-// neighbour rows (4 variables x MB members each) from HBM/L2 into LDS, then
-// lane (node, member) gathers its 31 neighbours from LDS.  Member blocks of
-// one tile are consecutive workgroups on the same XCD, so the tile's weights
-// come from HBM once and from L2 for the other member blocks.
+// LDS tiles: the device mesh is stored in a space-filling-curve node order
+// and cut into tiles of consecutive nodes whose stencils share few
+// neighbours; a tile's distinct neighbour states are copied into LDS once
+// and every stencil entry is read from there.  Used by the single-state tile
+// kernels (k_stage_ptile, k_stage_tile32) and the member-lane ensemble
+// kernel (k_stage_tspec).
 struct DevTiles {
     const int *tile_start;          // [ntiles + 1] first node of each tile
     const int *ustart;              // [ntiles + 1] offsets into ulist
@@ -835,75 +720,6 @@ struct DevTiles {
     int ntiles;
     int nmb;                        // member blocks per tile (B / MB)
 };
-
-template <int STAGE, bool EXACT, int MB, int NNBR>
-__global__ __launch_bounds__(512) void k_stage_tiled(DevMesh m, DevTiles tl, DevStage st)
-{
-    static_assert(MB >= 2 && MB <= 64 && (MB & (MB - 1)) == 0, "MB power of two");
-    extern __shared__ double rows[];   // [ucount][4][MB]
-    const int lb = xcd_block(st.nblk);
-    const int tile = lb / tl.nmb;
-    const int mb = lb - tile * tl.nmb;
-    if (tile >= tl.ntiles || mb * MB >= *st.active)
-        return;   // uniform over the workgroup
-    const int t0 = tl.tile_start[tile];
-    const int tcount = tl.tile_start[tile + 1] - t0;
-    const int u0 = tl.ustart[tile];
-    const int ucount = tl.ustart[tile + 1] - u0;
-    const long B = st.members;
-    const int col0 = mb * MB;
-    const double *__restrict__ src = st.src;
-    const int total = ucount * 4 * MB;
-    for (int i = (int)threadIdx.x; i < total; i += (int)blockDim.x) {
-        const int r = i / MB;
-        const int c = i & (MB - 1);
-        const int u = tl.ulist[u0 + (r >> 2)];
-        rows[i] = src[(4L * u + (r & 3)) * B + col0 + c];
-    }
-    __syncthreads();
-    const int nl = (int)threadIdx.x / MB;
-    const int ml = (int)threadIdx.x & (MB - 1);
-    if (nl >= tcount)
-        return;   // no barrier follows
-    const int node = t0 + nl;
-    const int nn = NNBR > 0 ? NNBR : m.nnbr;
-    const long ld = slot_stride(m);
-    const long lxs = m.ld;   // tile-local ids are slot-major [nnbr][n]
-    const unsigned short *__restrict__ lx = tl.lidx + node;
-    const double *__restrict__ wp = m.wts + 4L * slot_base<NNBR>(m, node);
-    Sums S = {};
-    auto slot = [&](int j) {
-        const double *sp = rows + (int)lx[j * lxs] * 4 * MB + ml;
-        const double su = sp[0], sv = sp[MB], sw = sp[2 * MB], sh = sp[3 * MB];
-        const double2 w01 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
-        const double2 w23 = *reinterpret_cast<const double2 *>(wp + 4L * j * ld + 2);
-        S.dudx = madd<EXACT>(S.dudx, w01.x, su); S.dvdx = madd<EXACT>(S.dvdx, w01.x, sv);
-        S.dwdx = madd<EXACT>(S.dwdx, w01.x, sw); S.dhdx = madd<EXACT>(S.dhdx, w01.x, sh);
-        S.dudy = madd<EXACT>(S.dudy, w01.y, su); S.dvdy = madd<EXACT>(S.dvdy, w01.y, sv);
-        S.dwdy = madd<EXACT>(S.dwdy, w01.y, sw); S.dhdy = madd<EXACT>(S.dhdy, w01.y, sh);
-        S.dudz = madd<EXACT>(S.dudz, w23.x, su); S.dvdz = madd<EXACT>(S.dvdz, w23.x, sv);
-        S.dwdz = madd<EXACT>(S.dwdz, w23.x, sw); S.dhdz = madd<EXACT>(S.dhdz, w23.x, sh);
-        S.lu = madd<EXACT>(S.lu, w23.y, su); S.lv = madd<EXACT>(S.lv, w23.y, sv);
-        S.lw = madd<EXACT>(S.lw, w23.y, sw); S.lh = madd<EXACT>(S.lh, w23.y, sh);
-    };
-    if constexpr (NNBR > 0) {
-#pragma unroll 8
-        for (int j = 0; j < NNBR; ++j)
-            slot(j);
-    } else {
-#pragma unroll 4
-        for (int j = 0; j < nn; ++j)
-            slot(j);
-    }
-    const long e = 4L * node * B + col0 + ml;
-    const double u = src[e], v = src[e + B], w = src[e + 2 * B], h = src[e + 3 * B];
-    const Geo g = load_geo(m.geo, m.ld, node);
-    double F[4];
-    rhs_point(S, u, v, w, h, g, m.gh0, F);
-#pragma unroll
-    for (int var = 0; var < 4; ++var)
-        stage_update<STAGE, EXACT>(st, e + var * B, F[var]);
-}
 
 // 16 bytes per lane from global memory straight into LDS (gfx950
 // global_load_lds_dwordx4): lane l's bytes land at lds + 16 l, lds being

@@ -78,10 +78,12 @@ def test_bad_options_rejected_before_device(pkg):
     for field, bad in [("kernel", 7), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
                        ("slot_block", 8), ("members", 0), ("member_block", 3),
                        ("tiled", 2), ("tiled", -1), ("tile_nodes", 100), ("tile_nodes", -2),
-                       ("tile_rows", 8), ("tile_rows", 71)]:
+                       ("tile_rows", 8), ("tile_rows", 71), ("kernel", 1), ("kernel", 2)]:
         o = pkg.runtime.SweHipOptions(**dict(base, **{field: bad}))
         if field == "member_block":
             o.members = 8
+        if field == "tiled" and bad == 2:   # and a tiled block other than 32
+            o.tiled, o.members, o.member_block = 1, 64, 8
         out = C.c_void_p()
         assert L.swe_hip_create_ex(C.byref(s), C.byref(o), C.byref(out)) == -1, field
         assert out.value is None

@@ -90,7 +90,7 @@ def test_ensemble_tuner_covers_tiled_kernels(pkg):
     with pkg.runtime.HipContext(prob, order="hilbert") as base:
         mb, tiled, ms = base.tune_ensemble(64, steps=3)
         assert mb in (2, 4, 8, 16, 32, 64) and ms > 0.0
-        assert not tiled or mb in (4, 8, 16, 32)
+        assert not tiled or mb == 32
         with pkg.runtime.HipContext(prob, members=64, member_block=mb, tiled=tiled,
                                     share=base) as ctx:
             ctx.set_state(prob.H_init, 9)
@@ -123,14 +123,13 @@ def test_set_active_and_move_member(pkg):
         assert np.array_equal(solo.get_state(), got)
 
 
-@pytest.mark.parametrize("B,mb,order", [(8, 4, "morton"), (16, 8, "morton"), (32, 16, "morton"),
-                                        (64, 8, "morton"), (32, 32, "morton"),
-                                        (64, 32, "morton"), (96, 32, "morton"),
-                                        (16, 8, "hilbert"), (64, 32, "hilbert")])
+@pytest.mark.parametrize("B,mb,order", [(32, 32, "morton"), (64, 32, "morton"),
+                                        (96, 32, "morton"), (64, 32, "hilbert"),
+                                        (32, 32, "host")])
 def test_tiled_exact_members_equal_oracle(pkg, pyoracle, B, mb, order):
-    """LDS-staged kernels over a space-filling-curve-ordered device copy:
-    bitwise equal to the oracle (node order and staging never change a
-    node's arithmetic)."""
+    """The member-lane LDS-tile kernel over any device node order: bitwise
+    equal to the oracle (node order and staging never change a node's
+    arithmetic)."""
     prob = pkg.host.Problem(pkg.mesh.case_path(2562, seed=1), use_rcm=True, layout="sfdl")
     print(f"case: tiled exact B={B} mb={mb} order={order} N=2562", flush=True)
     H0 = _members(pkg, prob, B)
@@ -163,7 +162,7 @@ def test_member_lane_tiles_host_order_substeps_exact(pkg, pyoracle):
             assert np.array_equal(ctx.get_state(m), orc.run(H0[m], 3)), m
 
 
-@pytest.mark.parametrize("mb,order", [(16, "morton"), (32, "morton"), (32, "hilbert")])
+@pytest.mark.parametrize("mb,order", [(32, "morton"), (32, "hilbert")])
 def test_tiled_fast_composition_and_active(pkg, mb, order):
     prob = pkg.host.Problem(pkg.mesh.case_path(10242, seed=1), use_rcm=True, layout="cfdl")
     print(f"case: tiled fast B=64 mb={mb} order={order} N=10242 vs solo", flush=True)
@@ -187,10 +186,8 @@ def test_tiled_fast_composition_and_active(pkg, mb, order):
         assert np.array_equal(solo.get_state(), got15)
 
 
-@pytest.mark.parametrize("kernel,pf,sblk,order", [("quad", 0, 0, "morton"), ("quad", 8, 0, "morton"),
-                                                 ("lane", 0, 0, "morton"), ("lane", 4, 0, "morton"),
-                                                 ("quad", 0, 16, "morton"), ("lane", 4, 16, "morton"),
-                                                 ("quadw", 0, 0, "morton"),
+@pytest.mark.parametrize("kernel,pf,sblk,order", [("quadw", 0, 0, "morton"),
+                                                 ("quadw", 8, 0, "host"),
                                                  ("quadw", 4, 16, "morton"),
                                                  ("pair", 0, 0, "morton"), ("pair", 0, 16, "morton"),
                                                  ("pair", 0, 0, "hilbert"), ("tile", 0, 0, "hilbert"),
@@ -208,7 +205,7 @@ def test_single_state_variants_exact(pkg, pyoracle, kernel, pf, sblk, order):
 
 @pytest.mark.parametrize("kernel,order,sblk", [("pair", "host", 0), ("pair", "morton", 16),
                                                ("pair", "hilbert", 0), ("quadw", "morton", 16),
-                                               ("lane", "host", 0), ("tile", "hilbert", 0),
+                                               ("quadw", "host", 0), ("tile", "hilbert", 0),
                                                ("tile", "host", 16)])
 def test_single_state_variants_fast(pkg, pyoracle, kernel, order, sblk):
     """Fast single-state kernels within 1e-12 of the oracle after 50 steps."""
@@ -222,7 +219,7 @@ def test_single_state_variants_fast(pkg, pyoracle, kernel, order, sblk):
         assert (err <= 1e-12).all(), err
 
 
-@pytest.mark.parametrize("tiled,mb", [(False, 64), (False, 8), (True, 8), (True, 32)])
+@pytest.mark.parametrize("tiled,mb", [(False, 64), (False, 8), (True, 32)])
 def test_blocked_stencil_layout_members_exact(pkg, pyoracle, tiled, mb):
     """Node-blocked stencil arrays (slot_block=16, N=642 is not a multiple
     of 16) give the oracle's bits for the ensemble kernels too."""
