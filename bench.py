@@ -72,7 +72,7 @@ def parse_args():
     ap.add_argument("--members", type=int, default=1)
     ap.add_argument("--member-block", type=int, default=0)
     ap.add_argument("--kernel", default="auto",
-                    choices=["auto", "quad", "lane", "quadw", "pair", "tile", "oct"])
+                    choices=["auto", "quad", "lane", "quadw", "pair", "tile", "oct", "resident"])
     ap.add_argument("--prefetch", type=int, default=0)
     ap.add_argument("--order", default="hilbert", choices=["host", "morton", "hilbert"],
                     help="node order of the device copy (results are identical in any order; "
@@ -156,9 +156,10 @@ def dominant_kernel(args):
         return {"lane": "k_stage_lane32", "tile": "k_stage_tile32"}.get(k, "k_stage_split32")
     if args.members == 1:
         if args.kernel == "auto":
-            return "k_stage_pair"
+            # the library's auto rule (256 nodes per CU on the 256 CUs of MI355X)
+            return "k_steps_resident" if 20000 <= args.nodes <= 256 * 256 else "k_stage_pair"
         return {"quadw": "k_stage_quadw", "pair": "k_stage_pair",
-                "tile": "k_stage_ptile"}[args.kernel]
+                "tile": "k_stage_ptile", "resident": "k_steps_resident"}[args.kernel]
     if args.tiled:
         return "k_stage_tspec"
     return "k_stage_mpair" if args.member_block in (0, 64) and args.members % 64 == 0 \
@@ -188,6 +189,8 @@ def describe(opts):
         if opts["kernel"] == "tile":
             return (f"k_stage_ptile (pair mapping over LDS tiles of {opts['tile_nodes']} nodes, "
                     f"row staging {opts['prefetch']})")
+        if opts["kernel"] == "resident":
+            return "k_steps_resident (whole steps in one cooperative launch)"
         return "k_stage_pair"
     if opts["tiled"]:
         return (f"k_stage_tspec (member-lane LDS tiles, {opts['tile_nodes'] or 8} nodes x "

@@ -121,8 +121,10 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
  * the static data is reused (s may then be NULL): the static data stay
  * resident until the last context using them is destroyed.  Out-of-range
  * options fail with SWE_HIP_EINVAL before any device call. */
-#define SWE_HIP_KERNEL_AUTO 0   /* single state: PAIR with a 31-point
-                                   stencil, else QUADW                       */
+#define SWE_HIP_KERNEL_AUTO 0   /* single state: RESIDENT (FP64) from
+                                   20000 up to 256 nodes per compute unit,
+                                   else PAIR with a 31-point stencil, else
+                                   QUADW                                     */
 #define SWE_HIP_KERNEL_QUAD 1   /* FP32 single state: 4 lanes per node
                                    splitting the stencil slots (fast mode)   */
 #define SWE_HIP_KERNEL_LANE 2   /* FP32 single state: 1 lane per node        */
@@ -139,6 +141,23 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
                                    HILBERT node order for compact tiles)     */
 #define SWE_HIP_KERNEL_OCT 6    /* FP32 single state: 8 lanes per node
                                    splitting the stencil slots (fast mode)   */
+#define SWE_HIP_KERNEL_RESIDENT 7 /* FP64 single state, 31-point stencils, at
+                                   most 256 nodes per compute unit (65536
+                                   on MI355X): swe_hip_rk_steps runs all its
+                                   steps in one cooperative launch, one
+                                   workgroup per tile of consecutive device
+                                   nodes keeping the tile's weights,
+                                   positions, geometry, H and D in registers
+                                   and meeting the others at a grid barrier
+                                   after every stage; same arithmetic as
+                                   PAIR (bitwise, both modes).  A barrier
+                                   wait that times out (the state is then
+                                   invalid) is reported as SWE_HIP_EDEVICE
+                                   by the next swe_hip_synchronize,
+                                   swe_hip_get_state, swe_hip_query or
+                                   swe_hip_rk_steps after the stream has
+                                   drained.  Drop-in substeps and eval_rhs
+                                   use PAIR.                                */
 /* In SWE_HIP_FP32 contexts: LANE = one lane per node; TILE = one lane per
  * node over LDS tiles of tile_nodes (64, 128 or 256) consecutive device
  * nodes (31-point stencils; use a HILBERT node order); both sum every

@@ -1,0 +1,357 @@
+// Whole RK4 steps of one single-state mesh in one cooperative launch: each
+// workgroup owns one tile of consecutive nodes for the whole launch and
+// keeps everything static about them -- 31 x 4 weights, stencil positions,
+// geometry -- and its running H and D in registers; per stage it only
+// copies the tile's distinct neighbour states into LDS, evaluates the
+// stencil sums, the RHS and the stage update of its nodes, publishes their
+// new state and meets the other workgroups at a grid barrier.  The static
+// stencil data (1252 of the 1388 bytes a node costs one stage launch of the
+// stage kernels) is read once per launch instead of once per stage.
+//
+// Same sums, same order and same stage arithmetic as k_stage_pair and
+// k_stage_ptile (bitwise in exact mode and in fast mode); the RK stages and
+// constants of RK_substep (rk4_rbffd_swe.c:30-72) and the fused stage
+// kernels: stage 0 gathers H and writes Ka, stage 1 Ka -> Kb, stage 2
+// Kb -> Ka, stage 3 Ka -> H.
+//
+// Cross-workgroup visibility: the L2 caches of the 8 XCDs are not coherent
+// with each other, so every state access another workgroup depends on is
+// agent-coherent (buffer loads and stores with the sc1 cache policy; the
+// published states are small and live in the last-level cache) and the
+// barrier's counters are agent-scope atomics.  A barrier wait is bounded
+// (spin_limit polls): a workgroup that exceeds it raises the abort flag,
+// which every waiting workgroup sees, and all of them leave the kernel; the
+// host reports the launch as failed.
+#include "swe_ctx.h"
+
+#include <algorithm>
+
+namespace swe {
+
+// Grid barrier state, zeroed before the first launch of a context and when
+// the counters near 2^31 (they count on across launches: barrier k of a
+// launch is barrier bar0 + k of the state).  Workgroups arrive on the
+// counter of their group (blockIdx & 7: hardware workgroups are dealt
+// round-robin over the 8 XCDs), the last of a group on `total`, and the
+// last group releases every group's copy of the generation word.
+struct ResSync {
+    unsigned arrive[8][64];
+    unsigned gen[8][64];
+    unsigned total;
+    unsigned abort;
+    unsigned pad[62];
+};
+
+struct DevResident {
+    double *H, *Ka, *Kb;
+    ResSync *sync;
+    double c[4];        // K = H + c F (dt/2, dt/2, dt); stage 3: H += c D (dt/6)
+    double dcoef[4];    // D += dcoef F (0: D = F, 2, 2, 1)
+    int nsteps;
+    int bytes;          // bytes of each state buffer (buffer-resource bound)
+    int lds_rows;       // row halves the LDS holds (2 x the largest tile's rows)
+    unsigned bar0;      // barriers completed since the counters were cleared
+    unsigned spin_limit;
+};
+
+__device__ __forceinline__ unsigned ld_agent(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned add_agent(unsigned *p)
+{
+    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Barrier k of the launch (k = 0, 1, ...); false once any wait was aborted.
+__device__ bool res_barrier(ResSync *s, unsigned k, unsigned limit)
+{
+    __shared__ int ok;
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's state stores are complete
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned G = gridDim.x, grp = blockIdx.x & 7;
+        const unsigned groups = G < 8 ? G : 8;
+        const unsigned nx = G / 8 + (grp < G % 8 ? 1u : 0u);
+        if (add_agent(&s->arrive[grp][0]) == (k + 1) * nx - 1
+            && add_agent(&s->total) == (k + 1) * groups - 1)
+            for (unsigned y = 0; y < groups; ++y)
+                __hip_atomic_store(&s->gen[y][0], k + 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        int good = 1;
+        unsigned polls = 0;
+        while (ld_agent(&s->gen[grp][0]) < k + 1) {
+            if (++polls > limit || ld_agent(&s->abort)) {
+                __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc(const double *p, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), (short)0, bytes,
+                                             0x00020000);
+}
+
+constexpr int kSc1 = 16;   // cache policy: agent-coherent access
+
+__device__ __forceinline__ double2 load_sc1(__amdgpu_buffer_rsrc_t r, int off_bytes)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off_bytes, 0,
+                                                                              kSc1));
+}
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void store_sc1(__amdgpu_buffer_rsrc_t r, int off_bytes, double2 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off_bytes, 0, kSc1);
+}
+
+template <bool EXACT, int NNBR>
+__global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident(DevMesh m, DevTiles tl,
+                                                                      DevResident r)
+{
+    static_assert(NNBR > 0 && NNBR <= 31, "positions: 31 slots + the own row");
+    // LDS: the tile's rows [row][half] = (u, v), (w, h), then the byte
+    // offsets of those row halves in a state buffer
+    extern __shared__ double2 rows[];
+    const int tile = (int)blockIdx.x;   // one workgroup per tile
+    const int t0 = tl.tile_start[tile];
+    const int tcount = tl.tile_start[tile + 1] - t0;
+    const int u0 = tl.ustart[tile];
+    const int ucount = tl.ustart[tile + 1] - u0;
+    const int tid = (int)threadIdx.x;
+    const int nl = tid >> 1;
+    const int k = tid & 1;
+    const bool live = nl < tcount;
+    const int node = t0 + (live ? nl : tcount - 1);   // dead pairs stay whole for DPP
+
+    int *offs = reinterpret_cast<int *>(rows + r.lds_rows);
+    for (int i = tid; i < 2 * ucount; i += (int)blockDim.x)
+        offs[i] = 8 * (4 * tl.ulist[u0 + (i >> 1)] + 2 * (i & 1));
+    // this lane's weights (2k, 2k + 1) of every slot
+    const long ld = slot_stride(m);
+    const double *__restrict__ wp = m.wts + 4L * slot_base<NNBR>(m, node) + 2 * k;
+    double2 w[NNBR];
+    static_for<NNBR>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        w[j] = *reinterpret_cast<const double2 *>(wp + 4L * j * ld);
+    });
+    // tile positions of slots 16k .. 16k + 15 (two per dword)
+    const uint4 *pp = reinterpret_cast<const uint4 *>(tl.lidx_node + 32L * node) + 2 * k;
+    const uint4 pa = pp[0], pb = pp[1];
+    const unsigned pw[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+    double gv[(GEO_N + 1) / 2];
+#pragma unroll
+    for (int q = 0; q < (GEO_N + 1) / 2; ++q) {
+        const int f = 2 * q + k;
+        gv[q] = m.geo[(long)(f < GEO_N ? f : 0) * m.ld + node];
+    }
+    const int e = 8 * (4 * node + 2 * k);   // byte offset of this lane's two variables
+    double2 H = *reinterpret_cast<const double2 *>(r.H + 4L * node + 2 * k);
+    double2 D = make_double2(0.0, 0.0);
+    double2 K = H;   // the state the next stage gathers, this lane's part
+    unsigned bar = r.bar0;
+    bool alive = true;
+
+    for (int step = 0; step < r.nsteps && alive; ++step) {
+        static_for<4>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if (!alive)
+                return;
+            const double *src = s == 0 ? r.H : (s == 2 ? r.Kb : r.Ka);
+            double *out = s == 0 ? r.Ka : (s == 1 ? r.Kb : (s == 2 ? r.Ka : r.H));
+            const __amdgpu_buffer_rsrc_t rin = state_rsrc(src, r.bytes);
+            if constexpr (s == 0)
+                __syncthreads();   // offsets written (first step), rows free
+            for (int i = tid; i < 2 * ucount; i += (int)blockDim.x)
+                rows[i] = load_sc1(rin, offs[i]);
+            __syncthreads();
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;   // variable 2k   x (own0, own1, oth0, oth1)
+            double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;   // variable 2k+1 x (own0, own1, oth0, oth1)
+            static_for<NNBR>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const unsigned wv = (unsigned)pair_bcast_i<j / 16>((int)pw[(j % 16) / 2]);
+                const int pos = (int)((j & 1) ? (wv >> 16) : (wv & 0xffffu));
+                const double2 sv = rows[2 * pos + k];
+                const double o0 = pair_swap(w[j].x), o1 = pair_swap(w[j].y);
+                a0 = madd<EXACT>(a0, w[j].x, sv.x); a1 = madd<EXACT>(a1, w[j].y, sv.x);
+                a2 = madd<EXACT>(a2, o0, sv.x);     a3 = madd<EXACT>(a3, o1, sv.x);
+                b0 = madd<EXACT>(b0, w[j].x, sv.y); b1 = madd<EXACT>(b1, w[j].y, sv.y);
+                b2 = madd<EXACT>(b2, o0, sv.y);     b3 = madd<EXACT>(b3, o1, sv.y);
+            });
+            const double ax = k ? a2 : a0, ay = k ? a3 : a1, az = k ? a0 : a2, al = k ? a1 : a3;
+            const double bx = k ? b2 : b0, by = k ? b3 : b1, bz = k ? b0 : b2, bl = k ? b1 : b3;
+            const double pax = pair_swap(ax), pay = pair_swap(ay), paz = pair_swap(az),
+                         pal = pair_swap(al);
+            const double pbx = pair_swap(bx), pby = pair_swap(by), pbz = pair_swap(bz),
+                         pbl = pair_swap(bl);
+            Sums S;
+            S.dudx = k ? pax : ax; S.dudy = k ? pay : ay; S.dudz = k ? paz : az; S.lu = k ? pal : al;
+            S.dvdx = k ? pbx : bx; S.dvdy = k ? pby : by; S.dvdz = k ? pbz : bz; S.lv = k ? pbl : bl;
+            S.dwdx = k ? ax : pax; S.dwdy = k ? ay : pay; S.dwdz = k ? az : paz; S.lw = k ? al : pal;
+            S.dhdx = k ? bx : pbx; S.dhdy = k ? by : pby; S.dhdz = k ? bz : pbz; S.lh = k ? bl : pbl;
+            const double po0 = pair_swap(K.x), po1 = pair_swap(K.y);
+            const double u = k ? po0 : K.x, v = k ? po1 : K.y;
+            const double ww = k ? K.x : po0, h = k ? K.y : po1;
+            double gf[GEO_N];
+            static_for<GEO_N>([&](auto fc) {
+                constexpr int f = decltype(fc)::value;
+                gf[f] = pair_bcast<f & 1>(gv[f >> 1]);
+            });
+            Geo g;
+            g.x = gf[G_X]; g.y = gf[G_Y]; g.z = gf[G_Z]; g.f = gf[G_F]; g.ghm = gf[G_GHM];
+            g.pu0 = gf[G_PU]; g.pu1 = gf[G_PU + 1]; g.pu2 = gf[G_PU + 2];
+            g.pv0 = gf[G_PV]; g.pv1 = gf[G_PV + 1]; g.pv2 = gf[G_PV + 2];
+            g.pw0 = gf[G_PW]; g.pw1 = gf[G_PW + 1]; g.pw2 = gf[G_PW + 2];
+            g.gx = gf[G_GG]; g.gy = gf[G_GG + 1]; g.gz = gf[G_GG + 2];
+            double F[4];
+            rhs_point(S, u, v, ww, h, g, m.gh0, F);
+            const double f0 = k ? F[2] : F[0], f1 = k ? F[3] : F[1];
+            // the stage update of stage_update_pair, on registers
+            if constexpr (s == 0) {
+                D = make_double2(f0, f1);
+                K = make_double2(axpy<EXACT>(H.x, r.c[0], f0), axpy<EXACT>(H.y, r.c[0], f1));
+            } else if constexpr (s == 1 || s == 2) {
+                D = make_double2(axpy<EXACT>(D.x, r.dcoef[s], f0), axpy<EXACT>(D.y, r.dcoef[s], f1));
+                K = make_double2(axpy<EXACT>(H.x, r.c[s], f0), axpy<EXACT>(H.y, r.c[s], f1));
+            } else {
+                const double d0 = axpy<EXACT>(D.x, r.dcoef[3], f0);
+                const double d1 = axpy<EXACT>(D.y, r.dcoef[3], f1);
+                H = make_double2(axpy<EXACT>(H.x, r.c[3], d0), axpy<EXACT>(H.y, r.c[3], d1));
+                K = H;
+            }
+            if (live)
+                store_sc1(state_rsrc(out, r.bytes), e, K);
+            if (s < 3 || step + 1 < r.nsteps)
+                alive = res_barrier(r.sync, bar++, r.spin_limit);
+        });
+    }
+}
+
+__global__ void k_res_clear(unsigned *p, int words)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < words)
+        p[i] = 0u;
+}
+
+namespace {
+
+template <bool EX>
+void *resident_kernel()
+{
+    return reinterpret_cast<void *>(&k_steps_resident<EX, 31>);
+}
+
+}  // namespace
+
+int resident_sync_bytes() { return (int)sizeof(ResSync); }
+
+// LDS budget of both kernel instances and a check that one workgroup per
+// tile can be resident at once (the cooperative launch needs it).
+int resident_prepare(swe_hip_ctx *c, std::string *err)
+{
+    for (bool ex : {false, true}) {
+        void *fn = ex ? resident_kernel<true>() : resident_kernel<false>();
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c->res_lds)
+            != hipSuccess) {
+            *err = "resident kernel: LDS budget";
+            return SWE_HIP_EDEVICE;
+        }
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, c->res_block, c->res_lds)
+                != hipSuccess
+            || (long)per_cu * c->ncu < c->tiles.ntiles) {
+            *err = "resident kernel: the launch cannot be co-resident on this device";
+            return SWE_HIP_EDEVICE;
+        }
+    }
+    return 0;
+}
+
+int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err);
+
+// n fused RK steps in cooperative launches of at most 2^20 steps each.  The
+// launches are not waited for: each is followed by an asynchronous copy of
+// the abort flag into pinned host memory, read by resident_check once the
+// stream has drained.
+int resident_steps(swe_hip_ctx *c, int nsteps, std::string *err)
+{
+    if (c->res_pending && hipStreamQuery(c->stream) == hipSuccess)
+        if (int rc = resident_check(c, err))
+            return rc;
+    for (int done = 0; done < nsteps;) {
+        const int n = std::min(nsteps - done, 1 << 20);
+        if (int rc = resident_launch(c, n, err))
+            return rc;
+        done += n;
+    }
+    return 0;
+}
+
+int resident_check(swe_hip_ctx *c, std::string *err)
+{
+    if (!c->res_pending)
+        return 0;
+    c->res_pending = false;
+    if (*c->h_res_abort) {
+        *c->h_res_abort = 0;
+        c->res_barriers = 0;   // the counters are inconsistent: clear them next time
+        *err = "resident kernel: a grid barrier wait exceeded its limit (the state is invalid)";
+        return SWE_HIP_EDEVICE;
+    }
+    return 0;
+}
+
+int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err)
+{
+    const long long barriers = 4LL * nsteps - 1;
+    if (c->res_barriers == 0 || (c->res_barriers + barriers + 1) * 32 > (1LL << 31)) {
+        const int words = (int)(sizeof(ResSync) / sizeof(unsigned));
+        k_res_clear<<<(words + 255) / 256, 256, 0, c->stream>>>(
+            reinterpret_cast<unsigned *>(c->d_res_sync), words);
+        c->res_barriers = 0;
+    }
+    DevMesh m = mesh_of(c);
+    DevTiles tl = tiles_of(c, 1);
+    DevResident r{};
+    r.H = c->d_H;
+    r.Ka = c->d_Ka;
+    r.Kb = c->d_Kb;
+    r.sync = reinterpret_cast<ResSync *>(c->d_res_sync);
+    const double dt = c->dt;
+    r.c[0] = dt / 2.0; r.c[1] = dt / 2.0; r.c[2] = dt; r.c[3] = dt / 6.0;
+    r.dcoef[0] = 0.0; r.dcoef[1] = 2.0; r.dcoef[2] = 2.0; r.dcoef[3] = 1.0;
+    r.nsteps = nsteps;
+    r.bytes = (int)(state_count(c) * (long)sizeof(double));
+    r.spin_limit = 1u << 22;
+    r.lds_rows = 2 * c->tiles.umax_used;
+    r.bar0 = (unsigned)c->res_barriers;
+    void *args[] = {&m, &tl, &r};
+    void *fn = exact(c) ? resident_kernel<true>() : resident_kernel<false>();
+    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(c->tiles.ntiles), dim3(c->res_block), args,
+                                              (unsigned)c->res_lds, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->h_res_abort, &r.sync->abort, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) {
+        c->res_barriers = 0;
+        *err = std::string("resident kernel launch: ") + hipGetErrorString(e);
+        return SWE_HIP_EDEVICE;
+    }
+    c->res_barriers += barriers;
+    c->res_pending = true;
+    return 0;
+}
+
+}  // namespace swe

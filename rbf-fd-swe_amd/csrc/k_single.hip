@@ -66,7 +66,9 @@ template <int STAGE, bool EX, int NN>
 void family_single(const swe_hip_ctx *c, const DevStage &st)
 {
     const DevMesh m = mesh_of(c);
-    if (c->kernel == SWE_HIP_KERNEL_PAIR)
+    // the resident context's single stages (drop-in substeps, RHS) are the
+    // pair mapping's
+    if (c->kernel == SWE_HIP_KERNEL_PAIR || c->kernel == SWE_HIP_KERNEL_RESIDENT)
         launch_pair<STAGE, EX, NN>(c, m, st);
     else if (c->kernel == SWE_HIP_KERNEL_TILE && c->tiles.ntiles > 0)
         launch_ptile<STAGE, EX, NN>(c, m, st);

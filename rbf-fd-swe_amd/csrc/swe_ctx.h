@@ -20,6 +20,13 @@ constexpr int kBlock = 256;            // threads per workgroup, stage kernels
 // two on 65 CUs and one on the rest).
 constexpr int kSingleBlock = 64;
 constexpr int kGraphCount = 4;         // captured step graphs per context
+// Resident whole-step kernel (k_resident.hip): one tile of <= 32 x
+// kResMaxWaves nodes per compute unit, whose stencils touch <= kResMaxRows
+// distinct nodes (100 KB of LDS; a band of 161 nodes in RCM order touches
+// ~1100 at N = 40962, a Hilbert-ordered tile ~400).
+constexpr int kResMaxWaves = 8;
+constexpr int kResMaxRows = 2560;
+constexpr int kResAutoMinNodes = 20000;   // SWE_HIP_KERNEL_AUTO picks it from here
 
 // Host index macros of the reference (swe_config.h:21-54) at run time.
 struct HostLayout {
@@ -66,6 +73,7 @@ struct SweTiles {
     int *d_tile_start = nullptr, *d_ustart = nullptr, *d_ulist = nullptr;
     unsigned short *d_lidx = nullptr;
     unsigned short *d_lidx_node = nullptr;   // [n][32] node-major copy of d_lidx
+    int umax_used = 0;                       // most distinct rows of one tile
     int *d_ulist_tile = nullptr;             // [ntiles][kTmemStride] (member-lane kernel)
     double *d_wts_node = nullptr;            // [n][32][4] (member-lane kernel)
     double *d_geo_node = nullptr;            // [n][32] (member-lane kernel)
@@ -97,6 +105,12 @@ struct swe_hip_ctx {
     int *d_active = nullptr;     // member slots advanced by the ensemble kernel
     SweTiles tiles;              // LDS-staged ensemble kernel (ntiles == 0: unused)
     hipGraphExec_t graphs[swe::kGraphCount] = {};   // 64, 16, 4 and 1 RK steps
+    // resident whole-step kernel (kernel == SWE_HIP_KERNEL_RESIDENT)
+    int res_block = 0, res_lds = 0;   // threads per workgroup, dynamic LDS bytes
+    void *d_res_sync = nullptr;       // grid-barrier counters
+    unsigned *h_res_abort = nullptr;  // pinned: barrier time-out flag of the last launch
+    long long res_barriers = 0;       // barriers counted since the counters were cleared
+    bool res_pending = false;         // a launch's flag is not checked yet
     bool timing = false;
     swe_hip_timers acc{};
     long long bytes = 0;         // state buffers
@@ -133,6 +147,15 @@ inline DevMesh mesh_of(const swe_hip_ctx *c)
 // Records msg as the calling thread's last error (swe_hip_last_error) and
 // returns code.
 int set_error(int code, const std::string &msg);
+
+// Resident whole-step kernel (k_resident.hip): bytes of its barrier state,
+// LDS/co-residency set-up once the tiles are built, n steps (asynchronous),
+// and the check of the last launch's barrier time-out flag, to be called
+// once the context's stream has drained.
+int resident_sync_bytes();
+int resident_prepare(swe_hip_ctx *c, std::string *err);
+int resident_steps(swe_hip_ctx *c, int nsteps, std::string *err);
+int resident_check(swe_hip_ctx *c, std::string *err);
 
 // Looks up this device's no-allocation scratch address (swe_noalloc.hip);
 // the device must be current.  On failure *err says why.

@@ -138,6 +138,20 @@ int swe::set_error(int code, const std::string &msg) { return fail(code, msg); }
 
 namespace {
 
+// After the context's stream has drained: a grid-barrier time-out of the
+// last resident launch (its state is invalid) becomes this call's error.
+int check_resident(swe_hip_ctx *c)
+{
+    std::string why;
+    if (int rc = swe::resident_check(c, &why))
+        return fail(rc, why);
+    return 0;
+}
+
+}  // namespace
+
+namespace {
+
 #define HIPCHK(expr)                                                                   \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
@@ -298,9 +312,10 @@ void free_ctx(swe_hip_ctx *c)
                     c->f_H, c->f_Ka, c->f_Kb, c->f_D, c->f_F,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
                     c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile,
-                    c->tiles.d_wts_node, c->tiles.d_geo_node};
+                    c->tiles.d_wts_node, c->tiles.d_geo_node, c->d_res_sync};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->h_res_abort) (void)hipHostFree(c->h_res_abort);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     release_mesh(c->mesh);
     delete c;
@@ -630,6 +645,9 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
     t.ntiles = tile;
     t.tmax = tmax;
     t.umax = umax;
+    t.umax_used = 0;
+    for (int k = 0; k < tile; ++k)
+        t.umax_used = std::max(t.umax_used, ustart[k + 1] - ustart[k]);
     int rc = 0;
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_tile_start, sizeof(int) * tstart.size());
     if (!rc) rc = dev_alloc(&c->bytes, (void **)&t.d_ustart, sizeof(int) * ustart.size());
@@ -689,6 +707,7 @@ int time_options(swe_hip_ctx *base, const swe_hip_options &opt, int steps, doubl
     if (!rc && e == hipSuccess) rc = swe_hip_rk_steps(c, steps);
     if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
     if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+    if (!rc && e == hipSuccess) rc = check_resident(c);
     if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -762,7 +781,7 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         return fail(SWE_HIP_EINVAL, "static data is NULL");
     if (o->members < 1 || o->members > 4096)
         return fail(SWE_HIP_EINVAL, "members must be in 1..4096");
-    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_OCT)
+    if (o->kernel < SWE_HIP_KERNEL_AUTO || o->kernel > SWE_HIP_KERNEL_RESIDENT)
         return fail(SWE_HIP_EINVAL, "kernel must be one of SWE_HIP_KERNEL_*");
     if (o->tiled < 0 || o->tiled > 1)
         return fail(SWE_HIP_EINVAL, "tiled must be 0 or 1");
@@ -790,6 +809,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
                   || o->kernel == SWE_HIP_KERNEL_LANE))
         return fail(SWE_HIP_EINVAL, "FP64 single-state kernels are AUTO, PAIR, QUADW and TILE "
                                     "(QUAD, LANE and OCT are FP32 mappings)");
+    if (o->kernel == SWE_HIP_KERNEL_RESIDENT && (fp32 || o->members != 1 || o->tiled))
+        return fail(SWE_HIP_EINVAL, "the resident kernel takes FP64 single-state contexts");
     if (o->members == 1 && o->tile_nodes != 0 && o->tile_nodes != 64 && o->tile_nodes != 128
         && o->tile_nodes != 256)
         return fail(SWE_HIP_EINVAL, "tile_nodes of the single-state tile kernel must be 0, 64, "
@@ -851,11 +872,23 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->d_geo = m->d_geo;
     c->B = o->members;
     c->MB = mb;
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess || c->ncu <= 0)
+        c->ncu = 256;
     // auto: the pair mapping (half the address-unit work per node of the
-    // quad mapping) with one-wave workgroups is the fastest measured at
-    // N = 40962, 163842 and 655362; it needs a compile-time stencil width.
-    if (o->kernel == SWE_HIP_KERNEL_AUTO)
+    // quad mapping) with one-wave workgroups is the fastest stage kernel
+    // measured at N = 40962, 163842 and 655362; it needs a compile-time
+    // stencil width.  From kResAutoMinNodes up to 256 nodes per compute unit
+    // the resident whole-step kernel beats it (N = 40962: 6.3 against 10.2
+    // us per stage over 1000 steps); below, its barrier and per-stage
+    // memory round trip cost more than a stage launch (N = 10242: 5.3
+    // against 4.6 us).
+    if (o->kernel == SWE_HIP_KERNEL_AUTO) {
         c->kernel = m->nnbr == 31 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_QUADW;
+        if (!fp32 && o->members == 1 && !o->tiled && m->nnbr == 31 && m->n >= kResAutoMinNodes
+            && m->n <= 32 * kResMaxWaves * c->ncu)
+            c->kernel = SWE_HIP_KERNEL_RESIDENT;
+    }
     else if ((o->kernel == SWE_HIP_KERNEL_PAIR || o->kernel == SWE_HIP_KERNEL_TILE)
              && m->nnbr != 31)
         c->kernel = SWE_HIP_KERNEL_QUADW;   // both need a compile-time stencil width
@@ -879,9 +912,6 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     c->tile_nodes = o->tile_nodes;
     c->tile_rows = o->tile_rows;
     c->flags = o->flags;
-    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) !=
-            hipSuccess || c->ncu <= 0)
-        c->ncu = 256;
     int rc = 0;
     auto check = [&](hipError_t e, const char *what) {
         if (e != hipSuccess && !rc)
@@ -910,6 +940,32 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         if (!c->tile_nodes)
             c->tile_nodes = kPtileNodes;
         rc = build_tiles(c, c->tile_nodes, 4 * c->tile_nodes);
+    }
+    if (!rc && c->kernel == SWE_HIP_KERNEL_RESIDENT) {
+        // one tile of <= ceil(n / CUs) consecutive nodes per compute unit
+        const int tmax = (c->n + c->ncu - 1) / c->ncu;
+        const int waves = (tmax + 31) / 32;
+        if (c->nnbr != 31 || waves > kResMaxWaves)
+            rc = fail(SWE_HIP_EINVAL, "the resident kernel takes 31-point stencils and at most "
+                                      + std::to_string(32 * kResMaxWaves)
+                                      + " nodes per compute unit");
+        if (!rc) {
+            c->res_block = 64 * waves;
+            rc = build_tiles(c, tmax, kResMaxRows);
+        }
+        if (!rc && c->tiles.ntiles > c->ncu)
+            rc = fail(SWE_HIP_EINVAL, "the resident kernel's tiles exceed one per compute unit");
+        if (!rc) {
+            c->res_lds = 2 * c->tiles.umax_used * (int)(sizeof(double2) + sizeof(int));
+            rc = dev_alloc(&c->bytes, &c->d_res_sync, (size_t)resident_sync_bytes());
+        }
+        if (!rc)
+            check(hipHostMalloc((void **)&c->h_res_abort, sizeof(unsigned)), "hipHostMalloc");
+        if (!rc) {
+            std::string why;
+            if (int e = resident_prepare(c, &why))
+                rc = fail(e, why);
+        }
     }
     if (!rc && o->tiled) {
         int lds_max = 0;
@@ -1030,6 +1086,8 @@ int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
                 add(0, SWE_HIP_KERNEL_TILE, 0, 0, tn, 0);
     } else if (members == 1) {
         add(0, SWE_HIP_KERNEL_PAIR, 0, 0, 0, 0);
+        if (base->nnbr == 31 && base->n <= 32 * kResMaxWaves * base->ncu)
+            add(0, SWE_HIP_KERNEL_RESIDENT, 0, 0, 0, 0);
         if (base->nnbr == 31)
             for (int tn : {64, 128, 256})
                 for (int pf : {0, 4, 8})
@@ -1089,8 +1147,11 @@ int swe_hip_query(swe_hip_ctx *c)
     if (!c)
         return fail(SWE_HIP_EINVAL, "ctx is NULL");
     hipError_t e = hipStreamQuery(c->stream);
-    if (e == hipSuccess)
+    if (e == hipSuccess) {
+        if (int rc = check_resident(c))
+            return rc;
         return 1;
+    }
     if (e == hipErrorNotReady)
         return 0;
     return fail(SWE_HIP_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
@@ -1162,6 +1223,8 @@ static int from_device_member(swe_hip_ctx *c, int member, const double *src, dou
                               c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (int rc = check_resident(c))
+        return rc;
     const std::vector<int> &perm = c->mesh->perm;
     for (int d = 0; d < n; ++d) {
         const int i = perm.empty() ? d : perm[d];
@@ -1195,6 +1258,8 @@ static int from_device32(swe_hip_ctx *c, const float *src, float *Hhost)
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipMemcpyAsync(t.data(), src, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (int rc = check_resident(c))
+        return rc;
     const std::vector<int> &perm = c->mesh->perm;
     for (int d = 0; d < n; ++d) {
         const int i = perm.empty() ? d : perm[d];
@@ -1392,6 +1457,12 @@ int swe_hip_rk_steps(swe_hip_ctx *c, int nsteps)
     if (!c || nsteps < 0)
         return fail(SWE_HIP_EINVAL, "rk_steps: bad ctx or nsteps");
     HIPCHK(hipSetDevice(c->dev));
+    if (c->kernel == SWE_HIP_KERNEL_RESIDENT) {
+        std::string why;
+        if (int rc = resident_steps(c, nsteps, &why))
+            return fail(rc, why);
+        return 0;
+    }
     // every graph is captured on the first call, so no later (timed) call
     // pays for a capture
     for (int g = 0; g < kGraphCount; ++g)
@@ -1429,7 +1500,7 @@ int swe_hip_synchronize(swe_hip_ctx *c)
         return fail(SWE_HIP_EINVAL, "ctx is NULL");
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipStreamSynchronize(c->stream));
-    return 0;
+    return check_resident(c);
 }
 
 }  // extern "C"

@@ -415,6 +415,15 @@ __device__ __forceinline__ double pair_bcast(double v)
     return __hiloint2double(hi, lo);
 }
 
+// DPP row_newbcast (64-bit moves): every lane of a 16-lane row takes lane
+// K's value of that row.
+template <int K>
+__device__ __forceinline__ double row_bcast64(double v)
+{
+    static_assert(K >= 0 && K < 16, "lane within a row");
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xF, 0xF, false);
+}
+
 // ---------------------------------------------------------------------------
 // Single state (B == 1), pair mapping: two lanes per node.  Lane k owns
 // variables 2k, 2k+1 and weights 2k, 2k+1 of every slot: per slot it issues
@@ -887,8 +896,9 @@ __device__ __forceinline__ int c_i32(const int *q)
 //  * the weights (31 x 32 B), geometry and 16-bit stencil positions of each
 //    wave's two nodes are staged in a wave-private LDS slice;
 //  * lane = member, two nodes per wave: per slot the lane reads its member's
-//    (u, v, w, h) (two ds_read2_b64) and its node's four weights (two b128
-//    broadcasts, or none with the DPP weight path of tmem_compute).
+//    (u, v, w, h) (two ds_read2_b64); its node's four weights come from
+//    registers by DPP broadcasts within each 16-lane row, read from LDS
+//    once per item.
 // Per member the sums, the RHS and the stage update are those of
 // k_stage_members (bitwise in exact mode).
 constexpr int kTmemTile = 8;    // nodes per tile: 4 waves x 2 nodes
@@ -1204,17 +1214,22 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
         return (int)((j & 1) ? (a >> 16) : (a & 0xffffu)) * 128;
     };
     const double *lrow = rows + ml;
-    const double2 *wrow = reinterpret_cast<const double2 *>(wl) + wid * 128 + half * 64;
-    // The LDS reads of a slot (member row, weights) run DEPTH slots ahead,
-    // in issue order, so waiting for slot j leaves the later slots in
-    // flight; slot j + DEPTH's reads are tied to the sums after slot j
-    // (bounded live ranges in exact mode too).
-    double2 ruv[DEPTH], rwh[DEPTH], rw01[DEPTH], rw23[DEPTH];
+    // Weights: lane k of each 16-lane row holds its node's four weights of
+    // slots k and k + 16 (rows 0, 1: node A; rows 2, 3: node B), four LDS
+    // reads per item; slot j's weights reach the whole row by DPP
+    // (row_newbcast from lane j & 15), so the stencil loop reads only the
+    // member rows from LDS.
+    const double2 *wq =
+        reinterpret_cast<const double2 *>(wl) + wid * 128 + half * 64 + 2 * (lane & 15);
+    const double2 wlo01 = wq[0], wlo23 = wq[1], whi01 = wq[32], whi23 = wq[33];
+    // The LDS reads of a slot's member row run DEPTH slots ahead, in issue
+    // order, so waiting for slot j leaves the later slots in flight; slot
+    // j + DEPTH's reads are tied to the sums after slot j (bounded live
+    // ranges in exact mode too).
+    double2 ruv[DEPTH], rwh[DEPTH];
     auto issue = [&](int j, int off) {
         ruv[j % DEPTH] = make_double2(lrow[off], lrow[off + 32]);        // u, v
         rwh[j % DEPTH] = make_double2(lrow[off + 64], lrow[off + 96]);   // w, h
-        rw01[j % DEPTH] = wrow[2 * j];
-        rw23[j % DEPTH] = wrow[2 * j + 1];
     };
     Sums S = {};
     if constexpr (MFMA) {
@@ -1225,7 +1240,10 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
     static_for<NNBR>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const double2 uv = ruv[j % DEPTH], wh = rwh[j % DEPTH];
-        acc_slot<EXACT>(S, rw01[j % DEPTH], rw23[j % DEPTH], uv.x, uv.y, wh.x, wh.y);
+        const double2 wa = j < 16 ? wlo01 : whi01, wb = j < 16 ? wlo23 : whi23;
+        const double2 w01 = make_double2(row_bcast64<j & 15>(wa.x), row_bcast64<j & 15>(wa.y));
+        const double2 w23 = make_double2(row_bcast64<j & 15>(wb.x), row_bcast64<j & 15>(wb.y));
+        acc_slot<EXACT>(S, w01, w23, uv.x, uv.y, wh.x, wh.y);
         if constexpr (j + DEPTH < NNBR) {
             int off = row_off(j + DEPTH);
             asm volatile(""

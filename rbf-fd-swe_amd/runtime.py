@@ -17,7 +17,8 @@ _ip = C.POINTER(C.c_int)
 SWE_HIP_EXACT = 1
 SWE_HIP_FP32 = 2
 _fp = C.POINTER(C.c_float)
-KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4, "tile": 5, "oct": 6}
+KERNELS = {"auto": 0, "quad": 1, "lane": 2, "quadw": 3, "pair": 4, "tile": 5, "oct": 6,
+           "resident": 7}
 ORDERS = {"host": 0, "morton": 1, "hilbert": 2}
 
 # Every function include/swe_hip.h declares.

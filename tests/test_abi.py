@@ -75,7 +75,7 @@ def test_bad_options_rejected_before_device(pkg):
     L = pkg.runtime.lib()
     base = dict(members=1, flags=0, device=0, member_block=0, share=None, kernel=0,
                 prefetch=0, node_order=0, tiled=0, slot_block=0)
-    for field, bad in [("kernel", 7), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
+    for field, bad in [("kernel", 8), ("kernel", -1), ("prefetch", 2), ("node_order", 3),
                        ("slot_block", 8), ("members", 0), ("member_block", 3),
                        ("tiled", 2), ("tiled", -1), ("tile_nodes", 100), ("tile_nodes", -2),
                        ("tile_rows", 8), ("tile_rows", 71), ("kernel", 1), ("kernel", 2)]:
@@ -100,7 +100,9 @@ def test_fp32_kernel_options_rejected_before_device(pkg):
            (p32, dict(fp32=True, exact=True, kernel="quad")),
            (p32, dict(fp32=True, kernel="quadw")), (p32, dict(fp32=True, kernel="tile",
                                                             tile_nodes=100)),
-           (p64, dict(kernel="oct"))]
+           (p64, dict(kernel="oct")),
+           # the resident kernel is FP64 and single-state
+           (p32, dict(fp32=True, kernel="resident")), (p64, dict(kernel="resident", members=2))]
     for prob, kw in bad:
         with pytest.raises(pkg.runtime.SweHipError) as ei:
             pkg.runtime.HipContext(prob, **kw)

@@ -1,0 +1,78 @@
+"""The resident whole-step kernel (SWE_HIP_KERNEL_RESIDENT): all RK steps of
+a call in one cooperative launch, grid barriers between the stages.
+
+It keeps the arithmetic of the pair mapping (rk4_rbffd_swe.c:30-72 stages,
+:285-353 sums and RHS), so exact mode is bitwise equal to the oracle and
+fast mode bitwise equal to the stage kernels replayed from graphs, at every
+size, node order and stencil layout, over repeated launches.  A stale read
+of a neighbour's state across the XCDs' caches would change the bits."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _prob(pkg, n, seed=1, rcm=True, layout="cfdl"):
+    return pkg.host.Problem(pkg.mesh.case_path(n, seed=seed), use_rcm=rcm, layout=layout, simd=4)
+
+
+@pytest.mark.parametrize("n,order,slot_block", [(642, "host", 0), (2562, "hilbert", 0),
+                                                (2562, "host", 16)])
+def test_exact_bitwise_vs_oracle(pkg, pyoracle, n, order, slot_block):
+    prob = _prob(pkg, n)
+    print(f"case: resident exact N={n} order={order} slot_block={slot_block} 4+7 steps",
+          flush=True)
+    want = pyoracle.Oracle(prob).run(prob.H_init, 11)
+    with pkg.runtime.HipContext(prob, exact=True, kernel="resident", order=order,
+                                slot_block=slot_block) as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(4)
+        ctx.rk_steps(0)
+        ctx.rk_steps(7)
+        got = ctx.get_state()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,order,steps", [(2562, "hilbert", 300), (10242, "morton", 200),
+                                           (40962, "hilbert", 1000), (40962, "host", 100)])
+def test_fast_bitwise_vs_stage_kernels(pkg, n, order, steps):
+    prob = _prob(pkg, n)
+    print(f"case: resident fast N={n} order={order} {steps} steps", flush=True)
+    with pkg.runtime.HipContext(prob, kernel="pair", order=order) as ref:
+        ref.set_state(prob.H_init)
+        ref.rk_steps(steps)
+        want = ref.get_state()
+    with pkg.runtime.HipContext(prob, kernel="resident", order=order) as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(steps // 2)
+        ctx.rk_steps(steps - steps // 2)
+        got = ctx.get_state()
+    assert np.isfinite(got).all()
+    assert np.array_equal(got, want)
+
+
+def test_substeps_and_rhs_of_a_resident_context(pkg, pyoracle):
+    """Drop-in substeps and eval_rhs of a resident context run the pair
+    mapping; mixing them with resident steps keeps the bits."""
+    prob = _prob(pkg, 2562)
+    print("case: resident context, substeps + steps + RHS, exact", flush=True)
+    orc = pyoracle.Oracle(prob)
+    want = orc.run(prob.H_init, 5)
+    with pkg.runtime.HipContext(prob, exact=True, kernel="resident") as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(2)
+        for s in range(4):
+            ctx.rk_substep(s)
+        ctx.rk_steps(2)
+        got = ctx.get_state()
+        rhs = ctx.eval_rhs(got)
+    assert np.array_equal(got, want)
+    assert np.array_equal(rhs, orc.eval_rhs(want))
+
+
+def test_too_many_nodes_per_compute_unit_is_rejected(pkg):
+    prob = _prob(pkg, 163842)
+    print("case: resident kernel at N=163842 (above 256 nodes per CU)", flush=True)
+    with pytest.raises(pkg.runtime.SweHipError) as ei:
+        pkg.runtime.HipContext(prob, kernel="resident")
+    assert ei.value.code == -1
