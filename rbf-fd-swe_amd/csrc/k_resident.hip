@@ -1,12 +1,12 @@
 // Whole RK4 steps of one single-state mesh in one cooperative launch: each
 // workgroup owns one tile of consecutive nodes for the whole launch and
 // keeps everything static about them -- 31 x 4 weights, stencil positions,
-// geometry -- and its running H and D in registers; per stage it only
-// copies the tile's distinct neighbour states into LDS, evaluates the
-// stencil sums, the RHS and the stage update of its nodes, publishes their
-// new state and meets the other workgroups at a grid barrier.  The static
-// stencil data (1252 of the 1388 bytes a node costs one stage launch of the
-// stage kernels) is read once per launch instead of once per stage.
+// geometry -- and its running H and D in registers; per stage it waits for
+// the tiles it exchanges rows with, copies its tile's distinct neighbour
+// states into LDS, evaluates the stencil sums, the RHS and the stage update
+// of its nodes, publishes their new state and its own stage count.  The
+// static stencil data (1252 of the 1388 bytes a node costs one stage launch
+// of the stage kernels) is read once per launch instead of once per stage.
 //
 // Same sums, same order and same stage arithmetic as k_stage_pair and
 // k_stage_ptile (bitwise in exact mode and in fast mode); the RK stages and
@@ -17,40 +17,40 @@
 // Cross-workgroup visibility: the L2 caches of the 8 XCDs are not coherent
 // with each other, so every state access another workgroup depends on is
 // agent-coherent (buffer loads and stores with the sc1 cache policy; the
-// published states are small and live in the last-level cache) and the
-// barrier's counters are agent-scope atomics.  A barrier wait is bounded
-// (spin_limit polls): a workgroup that exceeds it raises the abort flag,
-// which every waiting workgroup sees, and all of them leave the kernel; the
-// host reports the launch as failed.
+// published states are small and live in the last-level cache), and the
+// stage counts are agent-scope atomics written after the state stores
+// have completed.  Every wait is bounded (spin_limit polls): a workgroup
+// that exceeds it raises the abort flag, which every waiting workgroup
+// sees, and all of them leave the kernel; the host reports the launch as
+// failed.
 #include "swe_ctx.h"
 
 #include <algorithm>
 
 namespace swe {
 
-// Grid barrier state, zeroed before the first launch of a context and when
-// the counters near 2^31 (they count on across launches: barrier k of a
-// launch is barrier bar0 + k of the state).  Workgroups arrive on the
-// counter of their group (blockIdx & 7: hardware workgroups are dealt
-// round-robin over the 8 XCDs), the last of a group on `total`, and the
-// last group releases every group's copy of the generation word.
-struct ResSync {
-    unsigned arrive[8][64];
-    unsigned gen[8][64];
-    unsigned total;
-    unsigned abort;
-    unsigned pad[62];
-};
+// Synchronisation state (unsigned words), zeroed before the first launch of
+// a context and when the counts near 2^31: word 0 is the abort flag, then
+// one 128-byte line per tile holding the number of stages the tile has
+// completed, counted on across launches.  Instead of a grid-wide barrier a
+// tile waits, before each stage, for the tiles it exchanges rows with (the
+// owners of the rows it reads and the tiles that read its rows): they have
+// then finished the previous stage, so the rows it reads are final and
+// nobody still reads the buffer it is about to overwrite.  Neighbouring
+// tiles are at most one stage apart and no wait depends on a later stage.
+constexpr int kResLine = 32;
 
 struct DevResident {
     double *H, *Ka, *Kb;
-    ResSync *sync;
+    unsigned *sync;     // [1 + ntiles][kResLine]
+    unsigned *host_abort;   // pinned host word, set on a time-out
+    const int *nbrs;    // [ntiles][kResMaxNbrs] tiles to wait for, -1 padded
     double c[4];        // K = H + c F (dt/2, dt/2, dt); stage 3: H += c D (dt/6)
     double dcoef[4];    // D += dcoef F (0: D = F, 2, 2, 1)
     int nsteps;
     int bytes;          // bytes of each state buffer (buffer-resource bound)
     int lds_rows;       // row halves the LDS holds (2 x the largest tile's rows)
-    unsigned bar0;      // barriers completed since the counters were cleared
+    unsigned stage0;    // stages completed before this launch
     unsigned spin_limit;
 };
 
@@ -59,40 +59,47 @@ __device__ __forceinline__ unsigned ld_agent(const unsigned *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ unsigned add_agent(unsigned *p)
+__device__ __forceinline__ void st_agent(unsigned *p, unsigned v)
 {
-    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Barrier k of the launch (k = 0, 1, ...); false once any wait was aborted.
-__device__ bool res_barrier(ResSync *s, unsigned k, unsigned limit)
+// Wave 0 polls the neighbour tiles' stage counts (one per lane) until every
+// one has reached `need`; false once any wait in the grid was aborted.
+__device__ bool res_wait(const DevResident &r, int tile, unsigned need)
 {
     __shared__ int ok;
-    __builtin_amdgcn_s_waitcnt(0);   // this wave's state stores are complete
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned G = gridDim.x, grp = blockIdx.x & 7;
-        const unsigned groups = G < 8 ? G : 8;
-        const unsigned nx = G / 8 + (grp < G % 8 ? 1u : 0u);
-        if (add_agent(&s->arrive[grp][0]) == (k + 1) * nx - 1
-            && add_agent(&s->total) == (k + 1) * groups - 1)
-            for (unsigned y = 0; y < groups; ++y)
-                __hip_atomic_store(&s->gen[y][0], k + 1, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64) {
+        const int u = r.nbrs[tile * kResMaxNbrs + (int)threadIdx.x];
+        const unsigned *f = r.sync + (long)(1 + (u >= 0 ? u : tile)) * kResLine;
         int good = 1;
         unsigned polls = 0;
-        while (ld_agent(&s->gen[grp][0]) < k + 1) {
-            if (++polls > limit || ld_agent(&s->abort)) {
-                __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (!__all(u < 0 || ld_agent(f) >= need)) {
+            if (++polls > r.spin_limit || ld_agent(r.sync)) {
+                if (threadIdx.x == 0) {
+                    st_agent(r.sync, 1u);
+                    __hip_atomic_store(r.host_abort, 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+                }
                 good = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        ok = good;
+        if (threadIdx.x == 0)
+            ok = good;
     }
     __syncthreads();
     return ok != 0;
+}
+
+// Every wave's state stores are complete, then the tile's count moves on.
+__device__ __forceinline__ void res_publish(const DevResident &r, int tile, unsigned done)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        st_agent(r.sync + (long)(1 + tile) * kResLine, done);
 }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc(const double *p, int bytes)
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident(DevMesh m,
     double2 H = *reinterpret_cast<const double2 *>(r.H + 4L * node + 2 * k);
     double2 D = make_double2(0.0, 0.0);
     double2 K = H;   // the state the next stage gathers, this lane's part
-    unsigned bar = r.bar0;
+    unsigned stage = r.stage0;
     bool alive = true;
 
     for (int step = 0; step < r.nsteps && alive; ++step) {
@@ -171,8 +178,8 @@ __global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident(DevMesh m,
             const double *src = s == 0 ? r.H : (s == 2 ? r.Kb : r.Ka);
             double *out = s == 0 ? r.Ka : (s == 1 ? r.Kb : (s == 2 ? r.Ka : r.H));
             const __amdgpu_buffer_rsrc_t rin = state_rsrc(src, r.bytes);
-            if constexpr (s == 0)
-                __syncthreads();   // offsets written (first step), rows free
+            if (!(alive = res_wait(r, tile, stage)))   // also: offsets written, rows free
+                return;
             for (int i = tid; i < 2 * ucount; i += (int)blockDim.x)
                 rows[i] = load_sc1(rin, offs[i]);
             __syncthreads();
@@ -232,8 +239,7 @@ __global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident(DevMesh m,
             }
             if (live)
                 store_sc1(state_rsrc(out, r.bytes), e, K);
-            if (s < 3 || step + 1 < r.nsteps)
-                alive = res_barrier(r.sync, bar++, r.spin_limit);
+            res_publish(r, tile, ++stage);
         });
     }
 }
@@ -255,7 +261,7 @@ void *resident_kernel()
 
 }  // namespace
 
-int resident_sync_bytes() { return (int)sizeof(ResSync); }
+long resident_sync_bytes(int ntiles) { return (1L + ntiles) * kResLine * sizeof(unsigned); }
 
 // LDS budget of both kernel instances and a check that one workgroup per
 // tile can be resident at once (the cooperative launch needs it).
@@ -282,9 +288,8 @@ int resident_prepare(swe_hip_ctx *c, std::string *err)
 int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err);
 
 // n fused RK steps in cooperative launches of at most 2^20 steps each.  The
-// launches are not waited for: each is followed by an asynchronous copy of
-// the abort flag into pinned host memory, read by resident_check once the
-// stream has drained.
+// launches are not waited for: a time-out sets a pinned host word, read by
+// resident_check once the stream has drained.
 int resident_steps(swe_hip_ctx *c, int nsteps, std::string *err)
 {
     if (c->res_pending && hipStreamQuery(c->stream) == hipSuccess)
@@ -306,7 +311,7 @@ int resident_check(swe_hip_ctx *c, std::string *err)
     c->res_pending = false;
     if (*c->h_res_abort) {
         *c->h_res_abort = 0;
-        c->res_barriers = 0;   // the counters are inconsistent: clear them next time
+        c->res_stages = 0;   // the counts are inconsistent: clear them next time
         *err = "resident kernel: a grid barrier wait exceeded its limit (the state is invalid)";
         return SWE_HIP_EDEVICE;
     }
@@ -315,12 +320,12 @@ int resident_check(swe_hip_ctx *c, std::string *err)
 
 int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err)
 {
-    const long long barriers = 4LL * nsteps - 1;
-    if (c->res_barriers == 0 || (c->res_barriers + barriers + 1) * 32 > (1LL << 31)) {
-        const int words = (int)(sizeof(ResSync) / sizeof(unsigned));
+    const long long stages = 4LL * nsteps;
+    if (c->res_stages == 0 || c->res_stages + stages > (1LL << 31)) {
+        const int words = (int)(resident_sync_bytes(c->tiles.ntiles) / sizeof(unsigned));
         k_res_clear<<<(words + 255) / 256, 256, 0, c->stream>>>(
             reinterpret_cast<unsigned *>(c->d_res_sync), words);
-        c->res_barriers = 0;
+        c->res_stages = 0;
     }
     DevMesh m = mesh_of(c);
     DevTiles tl = tiles_of(c, 1);
@@ -328,7 +333,9 @@ int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err)
     r.H = c->d_H;
     r.Ka = c->d_Ka;
     r.Kb = c->d_Kb;
-    r.sync = reinterpret_cast<ResSync *>(c->d_res_sync);
+    r.sync = reinterpret_cast<unsigned *>(c->d_res_sync);
+    r.host_abort = c->d_res_abort;
+    r.nbrs = c->d_res_nbrs;
     const double dt = c->dt;
     r.c[0] = dt / 2.0; r.c[1] = dt / 2.0; r.c[2] = dt; r.c[3] = dt / 6.0;
     r.dcoef[0] = 0.0; r.dcoef[1] = 2.0; r.dcoef[2] = 2.0; r.dcoef[3] = 1.0;
@@ -336,20 +343,17 @@ int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err)
     r.bytes = (int)(state_count(c) * (long)sizeof(double));
     r.spin_limit = 1u << 22;
     r.lds_rows = 2 * c->tiles.umax_used;
-    r.bar0 = (unsigned)c->res_barriers;
+    r.stage0 = (unsigned)c->res_stages;
     void *args[] = {&m, &tl, &r};
     void *fn = exact(c) ? resident_kernel<true>() : resident_kernel<false>();
-    hipError_t e = hipLaunchCooperativeKernel(fn, dim3(c->tiles.ntiles), dim3(c->res_block), args,
-                                              (unsigned)c->res_lds, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(c->h_res_abort, &r.sync->abort, sizeof(unsigned),
-                           hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(c->tiles.ntiles), dim3(c->res_block),
+                                                    args, (unsigned)c->res_lds, c->stream);
     if (e != hipSuccess) {
-        c->res_barriers = 0;
+        c->res_stages = 0;
         *err = std::string("resident kernel launch: ") + hipGetErrorString(e);
         return SWE_HIP_EDEVICE;
     }
-    c->res_barriers += barriers;
+    c->res_stages += stages;
     c->res_pending = true;
     return 0;
 }

@@ -27,6 +27,8 @@ constexpr int kGraphCount = 4;         // captured step graphs per context
 constexpr int kResMaxWaves = 8;
 constexpr int kResMaxRows = 2560;
 constexpr int kResAutoMinNodes = 20000;   // SWE_HIP_KERNEL_AUTO picks it from here
+constexpr int kResMaxNbrs = 64;           // tiles one tile exchanges rows with (one wave)
+constexpr int kResAutoMinSteps = 4;       // AUTO contexts: shorter calls replay stage graphs
 
 // Host index macros of the reference (swe_config.h:21-54) at run time.
 struct HostLayout {
@@ -108,8 +110,11 @@ struct swe_hip_ctx {
     // resident whole-step kernel (kernel == SWE_HIP_KERNEL_RESIDENT)
     int res_block = 0, res_lds = 0;   // threads per workgroup, dynamic LDS bytes
     void *d_res_sync = nullptr;       // grid-barrier counters
-    unsigned *h_res_abort = nullptr;  // pinned: barrier time-out flag of the last launch
-    long long res_barriers = 0;       // barriers counted since the counters were cleared
+    unsigned *h_res_abort = nullptr;  // pinned: wait time-out flag of the resident launches
+    unsigned *d_res_abort = nullptr;  // its device address
+    bool res_auto = false;            // chosen by AUTO: short calls use the stage graphs
+    long long res_stages = 0;         // stages counted since the counts were cleared
+    int *d_res_nbrs = nullptr;        // [ntiles][kResMaxNbrs] tiles each tile waits for
     bool res_pending = false;         // a launch's flag is not checked yet
     bool timing = false;
     swe_hip_timers acc{};
@@ -152,7 +157,7 @@ int set_error(int code, const std::string &msg);
 // LDS/co-residency set-up once the tiles are built, n steps (asynchronous),
 // and the check of the last launch's barrier time-out flag, to be called
 // once the context's stream has drained.
-int resident_sync_bytes();
+long resident_sync_bytes(int ntiles);
 int resident_prepare(swe_hip_ctx *c, std::string *err);
 int resident_steps(swe_hip_ctx *c, int nsteps, std::string *err);
 int resident_check(swe_hip_ctx *c, std::string *err);

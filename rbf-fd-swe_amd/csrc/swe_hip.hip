@@ -312,7 +312,7 @@ void free_ctx(swe_hip_ctx *c)
                     c->f_H, c->f_Ka, c->f_Kb, c->f_D, c->f_F,
                     c->tiles.d_tile_start, c->tiles.d_ustart, c->tiles.d_ulist,
                     c->tiles.d_lidx, c->tiles.d_lidx_node, c->tiles.d_ulist_tile,
-                    c->tiles.d_wts_node, c->tiles.d_geo_node, c->d_res_sync};
+                    c->tiles.d_wts_node, c->tiles.d_geo_node, c->d_res_sync, c->d_res_nbrs};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_res_abort) (void)hipHostFree(c->h_res_abort);
@@ -582,7 +582,8 @@ __global__ void k_pack_tile_nodes(DevMesh m, int n, const unsigned short *lidx_n
     geo_node[t] = g;
 }
 
-int build_tiles(swe_hip_ctx *c, int tmax, int umax)
+int build_tiles(swe_hip_ctx *c, int tmax, int umax, std::vector<int> *h_tstart = nullptr,
+                std::vector<int> *h_ustart = nullptr, std::vector<int> *h_ulist = nullptr)
 {
     const int n = c->n, W = c->nnbr;
     const std::vector<int> &nbr = c->mesh->h_nbr;
@@ -678,6 +679,46 @@ int build_tiles(swe_hip_ctx *c, int tmax, int umax)
     }
     if (rc)
         t.ntiles = 0;
+    if (!rc && h_tstart) {
+        *h_tstart = tstart;
+        *h_ustart = ustart;
+        *h_ulist = ulist;
+    }
+    return rc;
+}
+
+// Resident kernel: for every tile, the tiles it must wait for before a
+// stage -- owners of the rows it reads and tiles that read its rows.
+int resident_neighbours(swe_hip_ctx *c, const std::vector<int> &tstart,
+                        const std::vector<int> &ustart, const std::vector<int> &ulist)
+{
+    const int nt = (int)tstart.size() - 1;
+    std::vector<int> owner(c->n);
+    for (int t = 0; t < nt; ++t)
+        for (int i = tstart[t]; i < tstart[t + 1]; ++i)
+            owner[i] = t;
+    std::vector<std::vector<int>> nb(nt);
+    for (int t = 0; t < nt; ++t)
+        for (int k = ustart[t]; k < ustart[t + 1]; ++k) {
+            const int u = owner[ulist[k]];
+            if (u != t) {
+                nb[t].push_back(u);
+                nb[u].push_back(t);
+            }
+        }
+    std::vector<int> flat((size_t)nt * kResMaxNbrs, -1);
+    for (int t = 0; t < nt; ++t) {
+        std::sort(nb[t].begin(), nb[t].end());
+        nb[t].erase(std::unique(nb[t].begin(), nb[t].end()), nb[t].end());
+        if ((int)nb[t].size() > kResMaxNbrs)
+            return fail(SWE_HIP_EINVAL, "the resident kernel's tiles exchange rows with more than "
+                                        + std::to_string(kResMaxNbrs) + " other tiles");
+        std::copy(nb[t].begin(), nb[t].end(), flat.begin() + (size_t)t * kResMaxNbrs);
+    }
+    int rc = dev_alloc(&c->bytes, (void **)&c->d_res_nbrs, sizeof(int) * flat.size());
+    if (!rc && hipMemcpy(c->d_res_nbrs, flat.data(), sizeof(int) * flat.size(),
+                         hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(SWE_HIP_EDEVICE, "resident kernel: neighbour upload");
     return rc;
 }
 
@@ -886,8 +927,10 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     if (o->kernel == SWE_HIP_KERNEL_AUTO) {
         c->kernel = m->nnbr == 31 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_QUADW;
         if (!fp32 && o->members == 1 && !o->tiled && m->nnbr == 31 && m->n >= kResAutoMinNodes
-            && m->n <= 32 * kResMaxWaves * c->ncu)
+            && m->n <= 32 * kResMaxWaves * c->ncu) {
             c->kernel = SWE_HIP_KERNEL_RESIDENT;
+            c->res_auto = true;
+        }
     }
     else if ((o->kernel == SWE_HIP_KERNEL_PAIR || o->kernel == SWE_HIP_KERNEL_TILE)
              && m->nnbr != 31)
@@ -949,18 +992,27 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
             rc = fail(SWE_HIP_EINVAL, "the resident kernel takes 31-point stencils and at most "
                                       + std::to_string(32 * kResMaxWaves)
                                       + " nodes per compute unit");
+        std::vector<int> ts, us, ul;
         if (!rc) {
             c->res_block = 64 * waves;
-            rc = build_tiles(c, tmax, kResMaxRows);
+            rc = build_tiles(c, tmax, kResMaxRows, &ts, &us, &ul);
         }
         if (!rc && c->tiles.ntiles > c->ncu)
             rc = fail(SWE_HIP_EINVAL, "the resident kernel's tiles exceed one per compute unit");
+        if (!rc)
+            rc = resident_neighbours(c, ts, us, ul);
         if (!rc) {
             c->res_lds = 2 * c->tiles.umax_used * (int)(sizeof(double2) + sizeof(int));
-            rc = dev_alloc(&c->bytes, &c->d_res_sync, (size_t)resident_sync_bytes());
+            rc = dev_alloc(&c->bytes, &c->d_res_sync,
+                           (size_t)resident_sync_bytes(c->tiles.ntiles));
         }
         if (!rc)
             check(hipHostMalloc((void **)&c->h_res_abort, sizeof(unsigned)), "hipHostMalloc");
+        if (!rc) {
+            *c->h_res_abort = 0;
+            check(hipHostGetDevicePointer((void **)&c->d_res_abort, c->h_res_abort, 0),
+                  "hipHostGetDevicePointer");
+        }
         if (!rc) {
             std::string why;
             if (int e = resident_prepare(c, &why))
@@ -1457,7 +1509,9 @@ int swe_hip_rk_steps(swe_hip_ctx *c, int nsteps)
     if (!c || nsteps < 0)
         return fail(SWE_HIP_EINVAL, "rk_steps: bad ctx or nsteps");
     HIPCHK(hipSetDevice(c->dev));
-    if (c->kernel == SWE_HIP_KERNEL_RESIDENT) {
+    // a resident context chosen by AUTO replays the stage graphs for calls
+    // too short to repay a launch (same arithmetic, bitwise)
+    if (c->kernel == SWE_HIP_KERNEL_RESIDENT && (!c->res_auto || nsteps >= kResAutoMinSteps)) {
         std::string why;
         if (int rc = resident_steps(c, nsteps, &why))
             return fail(rc, why);

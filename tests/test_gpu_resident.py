@@ -51,6 +51,23 @@ def test_fast_bitwise_vs_stage_kernels(pkg, n, order, steps):
     assert np.array_equal(got, want)
 
 
+def test_auto_mixes_graph_and_resident_calls(pkg):
+    """AUTO picks the resident kernel at N=40962 and replays the stage graphs
+    for calls shorter than 4 steps; any mix of calls gives the same bits."""
+    prob = _prob(pkg, 40962)
+    print("case: auto at N=40962, calls of 1, 2, 3, 10 and 4 steps", flush=True)
+    with pkg.runtime.HipContext(prob, kernel="pair") as ref:
+        ref.set_state(prob.H_init)
+        ref.rk_steps(20)
+        want = ref.get_state()
+    with pkg.runtime.HipContext(prob) as ctx:
+        ctx.set_state(prob.H_init)
+        for k in (1, 2, 3, 10, 4):
+            ctx.rk_steps(k)
+        got = ctx.get_state()
+    assert np.array_equal(got, want)
+
+
 def test_substeps_and_rhs_of_a_resident_context(pkg, pyoracle):
     """Drop-in substeps and eval_rhs of a resident context run the pair
     mapping; mixing them with resident steps keeps the bits."""
