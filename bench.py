@@ -166,6 +166,12 @@ def dominant_kernel(args):
         else "k_stage_members"
 
 
+RESIDENT_NOTE = ("k_steps_resident runs every stage of a call in one launch: 'launch' here is "
+                 "one RK stage (launch time / 4 steps), 'achieved' the stage kernels' "
+                 "algorithmic bytes per stage over that time; the kernel reads the static "
+                 "stencil data (1252 of the 1388 B per node) once per launch, not per stage, "
+                 "so frac can exceed 1; its HBM traffic is 'traffic' (rocprofv3 counts)")
+
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_NODE_RHS = 32 * NNBR + 66   # 16 stencil sums of 31 terms + the pointwise RHS
 
@@ -428,7 +434,8 @@ def main():
     # outside this process; the stored count is reported with its source and
     # only when it was taken for the kernel this run selects
     traffic, traffic_source = None, None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_n{args.nodes}_b{args.members}.json")
+    tpath = os.path.join(ROOT, "profiles",
+                         f"traffic_{dominant_kernel(args)}_n{args.nodes}_b{args.members}.json")
     if os.path.exists(tpath) and not args.fp32:
         with open(tpath) as fp:
             rec = json.load(fp)
@@ -484,7 +491,7 @@ def main():
         "dtype": "f32" if args.fp32 else "f64",
         "data": "synthetic (seeded icosahedral mesh + Gaussian RBF-FD weights, TC5-like state)",
         "config": {"workload": f"RBF-FD SWE RK4, N={prob.n_nodes} nodes, stencil 31, "
-                               f"{args.members} member(s), fused stage kernels + HIP graphs "
+                               f"{args.members} member(s), {dominant_kernel(args)} "
                                "(the 40962-node, 1000-step single-member case timed against "
                                "the reference CPU path)",
                    "nodes": prob.n_nodes, "members": args.members, "layout": args.layout,
@@ -497,7 +504,9 @@ def main():
                      "traffic_source": traffic_source,
                      "kernel": dominant_kernel(args),
                      "bytes_per_launch": stage_bytes(N, args.members, args.fp32),
-                     "avg_launch_us": launch_s * 1e6},
+                     "avg_launch_us": launch_s * 1e6,
+                     "note": RESIDENT_NOTE if dominant_kernel(args) == "k_steps_resident"
+                     else "one launch = one fused RK stage"},
         "cpu_baseline": cpu,
         "cpu_baseline_sfdl": cpu_sfdl,
         "c3": c3,
