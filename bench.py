@@ -152,8 +152,10 @@ def dominant_kernel(args):
     """Name of the stage kernel the options select (the one timed)."""
     if args.fp32:
         k = args.kernel if args.kernel != "auto" else \
-            ("lane" if args.exact or args.nodes > 262144 else "pair")
-        return {"lane": "k_stage_lane32", "tile": "k_stage_tile32"}.get(k, "k_stage_split32")
+            ("resident" if 20000 <= args.nodes <= 256 * 256 else
+             "lane" if args.exact or args.nodes > 262144 else "pair")
+        return {"lane": "k_stage_lane32", "tile": "k_stage_tile32",
+                "resident": "k_steps_resident32"}.get(k, "k_stage_split32")
     if args.members == 1:
         if args.kernel == "auto":
             # the library's auto rule (256 nodes per CU on the 256 CUs of MI355X)
@@ -340,7 +342,9 @@ FP32_KERNELS = {"lane": "k_stage_lane32 (one lane per node)",
                 "pair": "k_stage_split32 (2 lanes per node split the stencil slots)",
                 "quad": "k_stage_split32 (4 lanes per node split the stencil slots)",
                 "oct": "k_stage_split32 (8 lanes per node split the stencil slots)",
-                "tile": "k_stage_tile32 (one lane per node over LDS tiles of {tn} nodes)"}
+                "tile": "k_stage_tile32 (one lane per node over LDS tiles of {tn} nodes)",
+                "resident": "k_steps_resident32 (whole steps in one cooperative launch, one lane "
+                            "per node)"}
 
 
 def fp32_leg(pkg, torch, nodes, steps, warmup):
@@ -366,7 +370,9 @@ def fp32_leg(pkg, torch, nodes, steps, warmup):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "bytes_per_launch": stage_bytes(prob.padded, 1, fp32=True),
-                         "avg_launch_us": launch_us},
+                         "avg_launch_us": launch_us,
+                         "note": RESIDENT_NOTE if opts["kernel"] == "resident"
+                         else "one launch = one fused RK stage"},
             "state_finite": finite}
 
 

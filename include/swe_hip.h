@@ -121,10 +121,11 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
  * the static data is reused (s may then be NULL): the static data stay
  * resident until the last context using them is destroyed.  Out-of-range
  * options fail with SWE_HIP_EINVAL before any device call. */
-#define SWE_HIP_KERNEL_AUTO 0   /* single state: RESIDENT (FP64) from
-                                   20000 up to 256 nodes per compute unit,
-                                   else PAIR with a 31-point stencil, else
-                                   QUADW                                     */
+#define SWE_HIP_KERNEL_AUTO 0   /* single state: RESIDENT from 20000 up to
+                                   256 nodes per compute unit (with a
+                                   Hilbert device order unless another is
+                                   asked for), else PAIR with a 31-point
+                                   stencil, else QUADW (FP32: see below)     */
 #define SWE_HIP_KERNEL_QUAD 1   /* FP32 single state: 4 lanes per node
                                    splitting the stencil slots (fast mode)   */
 #define SWE_HIP_KERNEL_LANE 2   /* FP32 single state: 1 lane per node        */
@@ -141,32 +142,36 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
                                    HILBERT node order for compact tiles)     */
 #define SWE_HIP_KERNEL_OCT 6    /* FP32 single state: 8 lanes per node
                                    splitting the stencil slots (fast mode)   */
-#define SWE_HIP_KERNEL_RESIDENT 7 /* FP64 single state, 31-point stencils, at
+#define SWE_HIP_KERNEL_RESIDENT 7 /* single state (FP64: pair mapping; FP32:
+                                   one lane per node, LANE's arithmetic),
+                                   31-point stencils, at
                                    most 256 nodes per compute unit (65536
                                    on MI355X): swe_hip_rk_steps runs all its
                                    steps in one cooperative launch, one
                                    workgroup per tile of consecutive device
                                    nodes keeping the tile's weights,
                                    positions, geometry, H and D in registers
-                                   and meeting the others at a grid barrier
-                                   after every stage; same arithmetic as
-                                   PAIR (bitwise, both modes).  A barrier
-                                   wait that times out (the state is then
+                                   and waiting, before every stage, for the
+                                   tiles it exchanges rows with; same
+                                   arithmetic as PAIR (FP32: LANE), bitwise
+                                   in both modes.  A wait
+                                   that times out (the state is then
                                    invalid) is reported as SWE_HIP_EDEVICE
                                    by the next swe_hip_synchronize,
                                    swe_hip_get_state, swe_hip_query or
                                    swe_hip_rk_steps after the stream has
                                    drained.  Drop-in substeps and eval_rhs
-                                   use PAIR.                                */
+                                   use PAIR (FP32: LANE).                   */
 /* In SWE_HIP_FP32 contexts: LANE = one lane per node; TILE = one lane per
  * node over LDS tiles of tile_nodes (64, 128 or 256) consecutive device
  * nodes (31-point stencils; use a HILBERT node order); both sum every
  * stencil in order (exact and fast mode).  PAIR, QUAD and OCT = 2, 4 or 8
  * lanes per node, each summing every 2nd, 4th or 8th stencil slot, the
- * partial sums added across the group (fast mode only).  AUTO = LANE in
- * exact mode, and in fast mode PAIR up to 262144 nodes (one lane per node
- * would leave the chip under-filled), LANE above.  QUADW is an FP64
- * mapping. */
+ * partial sums added across the group (fast mode only).  RESIDENT = the
+ * resident whole-step kernel with one lane per node (both modes).  AUTO =
+ * RESIDENT from 20000 up to 256 nodes per compute unit, else LANE in exact
+ * mode, and in fast mode PAIR up to 262144 nodes (one lane per node would
+ * leave the chip under-filled), LANE above.  QUADW is an FP64 mapping. */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */
 #define SWE_HIP_ORDER_MORTON 1  /* device node order = 3-D Morton curve      */
 #define SWE_HIP_ORDER_HILBERT 2 /* device node order = 3-D Hilbert curve     */

@@ -244,6 +244,147 @@ __global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident(DevMesh m,
     }
 }
 
+// FP32 (the reference's RUN_32BIT type), one lane per node: the sums,
+// RHS and stage update of k_stage_lane32 (bitwise equal to it in both
+// modes, and in exact mode to the reference's FP32 build), the tile's rows
+// as 16-byte float4 in LDS, the same neighbour waits.
+struct DevResident32 {
+    float *H, *Ka, *Kb;
+    unsigned *sync;
+    unsigned *host_abort;
+    const int *nbrs;
+    float c[3];         // K = H + c F: (float)(dt/2.0), (float)(dt/2.0), dt
+    double c6;          // stage 3: H += (dt/6.0) D in double
+    float dcoef[4];     // D += dcoef F (0: D = F, 2, 2, 1)
+    int nsteps;
+    int bytes;
+    int lds_rows;       // rows the LDS holds (the largest tile's)
+    unsigned stage0;
+    unsigned spin_limit;
+};
+
+__device__ __forceinline__ float4 load_sc1_f4(__amdgpu_buffer_rsrc_t r, int off_bytes)
+{
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off_bytes, 0,
+                                                                             kSc1));
+}
+
+__device__ __forceinline__ void store_sc1_f4(__amdgpu_buffer_rsrc_t r, int off_bytes, float4 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off_bytes, 0, kSc1);
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc32(const float *p, int bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, bytes, 0x00020000);
+}
+
+template <bool EXACT, int NNBR>
+__global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident32(DevMesh32 m, DevTiles tl,
+                                                                        DevResident32 r)
+{
+    static_assert(NNBR > 0 && NNBR <= 32, "positions: 32 16-bit slots per node");
+    extern __shared__ float4 rows32[];   // the tile's rows, then their byte offsets
+    const int tile = (int)blockIdx.x;
+    const int t0 = tl.tile_start[tile];
+    const int tcount = tl.tile_start[tile + 1] - t0;
+    const int u0 = tl.ustart[tile];
+    const int ucount = tl.ustart[tile + 1] - u0;
+    const int tid = (int)threadIdx.x;
+    const bool live = tid < tcount;
+    const int node = t0 + (live ? tid : tcount - 1);
+    int *offs = reinterpret_cast<int *>(rows32 + r.lds_rows);
+    for (int i = tid; i < ucount; i += (int)blockDim.x)
+        offs[i] = 16 * tl.ulist[u0 + i];
+    const long ld = m.sblk ? 16L : (long)m.ld;
+    const long sb = m.sblk ? ((long)(node >> 4) * NNBR << 4) + (node & 15) : (long)node;
+    const float *__restrict__ wp = m.wts + 4L * sb;
+    float4 w[NNBR];
+    static_for<NNBR>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        w[j] = *reinterpret_cast<const float4 *>(wp + 4L * j * ld);
+    });
+    unsigned pw[16];
+    {
+        const uint4 *pp = reinterpret_cast<const uint4 *>(tl.lidx_node + 32L * node);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = pp[q];
+            pw[4 * q] = v.x; pw[4 * q + 1] = v.y; pw[4 * q + 2] = v.z; pw[4 * q + 3] = v.w;
+        }
+    }
+    float g[GEO_N];
+#pragma unroll
+    for (int f = 0; f < GEO_N; ++f)
+        g[f] = m.geo[(long)f * m.ld + node];
+    const int e = 16 * node;
+    float4 H = *reinterpret_cast<const float4 *>(r.H + 4L * node);
+    float4 D = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 K = H;
+    unsigned stage = r.stage0;
+    bool alive = true;
+    DevResident sync_view{};   // the fields res_wait and res_publish read
+    sync_view.sync = r.sync;
+    sync_view.host_abort = r.host_abort;
+    sync_view.nbrs = r.nbrs;
+    sync_view.spin_limit = r.spin_limit;
+
+    for (int step = 0; step < r.nsteps && alive; ++step) {
+        static_for<4>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if (!alive)
+                return;
+            const float *src = s == 0 ? r.H : (s == 2 ? r.Kb : r.Ka);
+            float *out = s == 0 ? r.Ka : (s == 1 ? r.Kb : (s == 2 ? r.Ka : r.H));
+            const __amdgpu_buffer_rsrc_t rin = state_rsrc32(src, r.bytes);
+            if (!(alive = res_wait(sync_view, tile, stage)))   // also: offsets written, rows free
+                return;
+            for (int i = tid; i < ucount; i += (int)blockDim.x)
+                rows32[i] = load_sc1_f4(rin, offs[i]);
+            __syncthreads();
+            Sums32 S = {};
+            static_for<NNBR>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const unsigned pv = pw[j >> 1];
+                const float4 sv = rows32[(j & 1) ? (pv >> 16) : (pv & 0xffffu)];
+                const float4 wj = w[j];
+                S.dudx = madd32<EXACT>(S.dudx, wj.x, sv.x); S.dvdx = madd32<EXACT>(S.dvdx, wj.x, sv.y);
+                S.dwdx = madd32<EXACT>(S.dwdx, wj.x, sv.z); S.dhdx = madd32<EXACT>(S.dhdx, wj.x, sv.w);
+                S.dudy = madd32<EXACT>(S.dudy, wj.y, sv.x); S.dvdy = madd32<EXACT>(S.dvdy, wj.y, sv.y);
+                S.dwdy = madd32<EXACT>(S.dwdy, wj.y, sv.z); S.dhdy = madd32<EXACT>(S.dhdy, wj.y, sv.w);
+                S.dudz = madd32<EXACT>(S.dudz, wj.z, sv.x); S.dvdz = madd32<EXACT>(S.dvdz, wj.z, sv.y);
+                S.dwdz = madd32<EXACT>(S.dwdz, wj.z, sv.z); S.dhdz = madd32<EXACT>(S.dhdz, wj.z, sv.w);
+                S.lu = madd32<EXACT>(S.lu, wj.w, sv.x); S.lv = madd32<EXACT>(S.lv, wj.w, sv.y);
+                S.lw = madd32<EXACT>(S.lw, wj.w, sv.z); S.lh = madd32<EXACT>(S.lh, wj.w, sv.w);
+            });
+            float F[4];
+            rhs_point32(S, K.x, K.y, K.z, K.w, g, m.gh0, F);
+            // the stage update of stage_update32, on registers
+            if constexpr (s == 0) {
+                D = make_float4(F[0], F[1], F[2], F[3]);
+                K = make_float4(axpy32<EXACT>(H.x, r.c[0], F[0]), axpy32<EXACT>(H.y, r.c[0], F[1]),
+                                axpy32<EXACT>(H.z, r.c[0], F[2]), axpy32<EXACT>(H.w, r.c[0], F[3]));
+            } else {
+                D = make_float4(axpy32<EXACT>(D.x, r.dcoef[s], F[0]),
+                                axpy32<EXACT>(D.y, r.dcoef[s], F[1]),
+                                axpy32<EXACT>(D.z, r.dcoef[s], F[2]),
+                                axpy32<EXACT>(D.w, r.dcoef[s], F[3]));
+                if constexpr (s == 3) {
+                    H = make_float4(update_h32<EXACT>(H.x, r.c6, D.x), update_h32<EXACT>(H.y, r.c6, D.y),
+                                    update_h32<EXACT>(H.z, r.c6, D.z), update_h32<EXACT>(H.w, r.c6, D.w));
+                    K = H;
+                } else {
+                    K = make_float4(axpy32<EXACT>(H.x, r.c[s], F[0]), axpy32<EXACT>(H.y, r.c[s], F[1]),
+                                    axpy32<EXACT>(H.z, r.c[s], F[2]), axpy32<EXACT>(H.w, r.c[s], F[3]));
+                }
+            }
+            if (live)
+                store_sc1_f4(state_rsrc32(out, r.bytes), e, K);
+            res_publish(sync_view, tile, ++stage);
+        });
+    }
+}
+
 __global__ void k_res_clear(unsigned *p, int words)
 {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -254,9 +395,10 @@ __global__ void k_res_clear(unsigned *p, int words)
 namespace {
 
 template <bool EX>
-void *resident_kernel()
+void *resident_kernel(bool fp32)
 {
-    return reinterpret_cast<void *>(&k_steps_resident<EX, 31>);
+    return fp32 ? reinterpret_cast<void *>(&k_steps_resident32<EX, 31>)
+                : reinterpret_cast<void *>(&k_steps_resident<EX, 31>);
 }
 
 }  // namespace
@@ -268,7 +410,7 @@ long resident_sync_bytes(int ntiles) { return (1L + ntiles) * kResLine * sizeof(
 int resident_prepare(swe_hip_ctx *c, std::string *err)
 {
     for (bool ex : {false, true}) {
-        void *fn = ex ? resident_kernel<true>() : resident_kernel<false>();
+        void *fn = ex ? resident_kernel<true>(c->fp32) : resident_kernel<false>(c->fp32);
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c->res_lds)
             != hipSuccess) {
             *err = "resident kernel: LDS budget";
@@ -327,27 +469,52 @@ int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err)
             reinterpret_cast<unsigned *>(c->d_res_sync), words);
         c->res_stages = 0;
     }
-    DevMesh m = mesh_of(c);
     DevTiles tl = tiles_of(c, 1);
-    DevResident r{};
-    r.H = c->d_H;
-    r.Ka = c->d_Ka;
-    r.Kb = c->d_Kb;
-    r.sync = reinterpret_cast<unsigned *>(c->d_res_sync);
-    r.host_abort = c->d_res_abort;
-    r.nbrs = c->d_res_nbrs;
-    const double dt = c->dt;
-    r.c[0] = dt / 2.0; r.c[1] = dt / 2.0; r.c[2] = dt; r.c[3] = dt / 6.0;
-    r.dcoef[0] = 0.0; r.dcoef[1] = 2.0; r.dcoef[2] = 2.0; r.dcoef[3] = 1.0;
-    r.nsteps = nsteps;
-    r.bytes = (int)(state_count(c) * (long)sizeof(double));
-    r.spin_limit = 1u << 22;
-    r.lds_rows = 2 * c->tiles.umax_used;
-    r.stage0 = (unsigned)c->res_stages;
-    void *args[] = {&m, &tl, &r};
-    void *fn = exact(c) ? resident_kernel<true>() : resident_kernel<false>();
-    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(c->tiles.ntiles), dim3(c->res_block),
-                                                    args, (unsigned)c->res_lds, c->stream);
+    void *fn = exact(c) ? resident_kernel<true>(c->fp32) : resident_kernel<false>(c->fp32);
+    hipError_t e = hipSuccess;
+    if (c->fp32) {
+        DevMesh32 m = mesh32_of(c);
+        DevResident32 r{};
+        r.H = c->f_H;
+        r.Ka = c->f_Ka;
+        r.Kb = c->f_Kb;
+        r.sync = reinterpret_cast<unsigned *>(c->d_res_sync);
+        r.host_abort = c->d_res_abort;
+        r.nbrs = c->d_res_nbrs;
+        // the constants of the FP32 stage kernels (rk4_rbffd_swe.c:123-186)
+        const float dt = (float)c->dt;
+        r.c[0] = (float)(dt / 2.0); r.c[1] = (float)(dt / 2.0); r.c[2] = dt;
+        r.c6 = dt / 6.0;
+        r.dcoef[0] = 0.0f; r.dcoef[1] = 2.0f; r.dcoef[2] = 2.0f; r.dcoef[3] = 1.0f;
+        r.nsteps = nsteps;
+        r.bytes = (int)(state_count(c) * (long)sizeof(float));
+        r.spin_limit = 1u << 22;
+        r.lds_rows = c->tiles.umax_used;
+        r.stage0 = (unsigned)c->res_stages;
+        void *args[] = {&m, &tl, &r};
+        e = hipLaunchCooperativeKernel(fn, dim3(c->tiles.ntiles), dim3(c->res_block), args,
+                                       (unsigned)c->res_lds, c->stream);
+    } else {
+        DevMesh m = mesh_of(c);
+        DevResident r{};
+        r.H = c->d_H;
+        r.Ka = c->d_Ka;
+        r.Kb = c->d_Kb;
+        r.sync = reinterpret_cast<unsigned *>(c->d_res_sync);
+        r.host_abort = c->d_res_abort;
+        r.nbrs = c->d_res_nbrs;
+        const double dt = c->dt;
+        r.c[0] = dt / 2.0; r.c[1] = dt / 2.0; r.c[2] = dt; r.c[3] = dt / 6.0;
+        r.dcoef[0] = 0.0; r.dcoef[1] = 2.0; r.dcoef[2] = 2.0; r.dcoef[3] = 1.0;
+        r.nsteps = nsteps;
+        r.bytes = (int)(state_count(c) * (long)sizeof(double));
+        r.spin_limit = 1u << 22;
+        r.lds_rows = 2 * c->tiles.umax_used;
+        r.stage0 = (unsigned)c->res_stages;
+        void *args[] = {&m, &tl, &r};
+        e = hipLaunchCooperativeKernel(fn, dim3(c->tiles.ntiles), dim3(c->res_block), args,
+                                       (unsigned)c->res_lds, c->stream);
+    }
     if (e != hipSuccess) {
         c->res_stages = 0;
         *err = std::string("resident kernel launch: ") + hipGetErrorString(e);

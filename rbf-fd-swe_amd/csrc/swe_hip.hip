@@ -843,15 +843,16 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     if (fp32 && o->kernel == SWE_HIP_KERNEL_QUADW)
         return fail(SWE_HIP_EINVAL, "SWE_HIP_FP32 kernels are AUTO, LANE, PAIR, QUAD, OCT and TILE");
     if (fp32 && (o->flags & SWE_HIP_EXACT) && o->kernel != SWE_HIP_KERNEL_AUTO
-        && o->kernel != SWE_HIP_KERNEL_LANE && o->kernel != SWE_HIP_KERNEL_TILE)
+        && o->kernel != SWE_HIP_KERNEL_LANE && o->kernel != SWE_HIP_KERNEL_TILE
+        && o->kernel != SWE_HIP_KERNEL_RESIDENT)
         return fail(SWE_HIP_EINVAL, "exact SWE_HIP_FP32 contexts sum each stencil in order: "
-                                    "kernel AUTO, LANE or TILE");
+                                    "kernel AUTO, LANE, TILE or RESIDENT");
     if (!fp32 && (o->kernel == SWE_HIP_KERNEL_OCT || o->kernel == SWE_HIP_KERNEL_QUAD
                   || o->kernel == SWE_HIP_KERNEL_LANE))
         return fail(SWE_HIP_EINVAL, "FP64 single-state kernels are AUTO, PAIR, QUADW and TILE "
                                     "(QUAD, LANE and OCT are FP32 mappings)");
-    if (o->kernel == SWE_HIP_KERNEL_RESIDENT && (fp32 || o->members != 1 || o->tiled))
-        return fail(SWE_HIP_EINVAL, "the resident kernel takes FP64 single-state contexts");
+    if (o->kernel == SWE_HIP_KERNEL_RESIDENT && (o->members != 1 || o->tiled))
+        return fail(SWE_HIP_EINVAL, "the resident kernel takes single-state contexts");
     if (o->members == 1 && o->tile_nodes != 0 && o->tile_nodes != 64 && o->tile_nodes != 128
         && o->tile_nodes != 256)
         return fail(SWE_HIP_EINVAL, "tile_nodes of the single-state tile kernel must be 0, 64, "
@@ -893,8 +894,18 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         m = o->share->mesh;
         m->refs++;
     } else {
-        int rc = build_mesh(s, device, o->node_order, o->slot_block,
-                            (o->flags & SWE_HIP_FP32) != 0, &m);
+        // AUTO on a mesh the resident kernel will take: its tiles are
+        // compact in Hilbert order (N = 40962, FP32: 3.7 against 4.6 us per
+        // stage in the host's RCM order), and the order changes no result
+        int order = o->node_order;
+        int ncu = 0;
+        if (o->kernel == SWE_HIP_KERNEL_AUTO && order == SWE_HIP_ORDER_HOST && o->members == 1
+            && !o->tiled && s->Nnbr == 31 && s->compute_size >= kResAutoMinNodes
+            && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device)
+                   == hipSuccess
+            && s->compute_size <= 32 * kResMaxWaves * ncu)
+            order = SWE_HIP_ORDER_HILBERT;
+        int rc = build_mesh(s, device, order, o->slot_block, (o->flags & SWE_HIP_FP32) != 0, &m);
         if (rc)
             return rc;
     }
@@ -944,9 +955,16 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         // than 4 waves per SIMD (N = 40962: 5.7 against 10.8 us per launch),
         // one lane above (N = 655362: 108.5 against 112.8 us)
         const int fast_auto = c->n <= 262144 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_LANE;
-        const int k = o->kernel == SWE_HIP_KERNEL_AUTO
-                          ? ((o->flags & SWE_HIP_EXACT) ? SWE_HIP_KERNEL_LANE : fast_auto)
-                          : o->kernel;
+        int k = o->kernel == SWE_HIP_KERNEL_AUTO
+                    ? ((o->flags & SWE_HIP_EXACT) ? SWE_HIP_KERNEL_LANE : fast_auto)
+                    : o->kernel;
+        // the resident kernel (both modes; its single stages are the lane
+        // kernel's) over the same node range as in FP64
+        if (o->kernel == SWE_HIP_KERNEL_AUTO && m->nnbr == 31 && m->n >= kResAutoMinNodes
+            && m->n <= 32 * kResMaxWaves * c->ncu) {
+            k = SWE_HIP_KERNEL_RESIDENT;
+            c->res_auto = true;
+        }
         c->kernel = k;
         c->split32 = k == SWE_HIP_KERNEL_PAIR ? 2 : k == SWE_HIP_KERNEL_QUAD ? 4
                    : k == SWE_HIP_KERNEL_OCT ? 8 : 1;
@@ -985,10 +1003,11 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         rc = build_tiles(c, c->tile_nodes, 4 * c->tile_nodes);
     }
     if (!rc && c->kernel == SWE_HIP_KERNEL_RESIDENT) {
-        // one tile of <= ceil(n / CUs) consecutive nodes per compute unit
+        // one tile of <= ceil(n / CUs) consecutive nodes per compute unit;
+        // two lanes per node in FP64, one in FP32
         const int tmax = (c->n + c->ncu - 1) / c->ncu;
-        const int waves = (tmax + 31) / 32;
-        if (c->nnbr != 31 || waves > kResMaxWaves)
+        const int waves = fp32 ? (tmax + 63) / 64 : (tmax + 31) / 32;
+        if (c->nnbr != 31 || tmax > 32 * kResMaxWaves)
             rc = fail(SWE_HIP_EINVAL, "the resident kernel takes 31-point stencils and at most "
                                       + std::to_string(32 * kResMaxWaves)
                                       + " nodes per compute unit");
@@ -1002,7 +1021,8 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         if (!rc)
             rc = resident_neighbours(c, ts, us, ul);
         if (!rc) {
-            c->res_lds = 2 * c->tiles.umax_used * (int)(sizeof(double2) + sizeof(int));
+            c->res_lds = c->tiles.umax_used * (fp32 ? (int)(sizeof(float4) + sizeof(int))
+                                                    : 2 * (int)(sizeof(double2) + sizeof(int)));
             rc = dev_alloc(&c->bytes, &c->d_res_sync,
                            (size_t)resident_sync_bytes(c->tiles.ntiles));
         }
@@ -1136,6 +1156,8 @@ int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
         if (base->nnbr == 31)
             for (int tn : {64, 128, 256})
                 add(0, SWE_HIP_KERNEL_TILE, 0, 0, tn, 0);
+        if (base->nnbr == 31 && base->n <= 32 * kResMaxWaves * base->ncu)
+            add(0, SWE_HIP_KERNEL_RESIDENT, 0, 0, 0, 0);
     } else if (members == 1) {
         add(0, SWE_HIP_KERNEL_PAIR, 0, 0, 0, 0);
         if (base->nnbr == 31 && base->n <= 32 * kResMaxWaves * base->ncu)

@@ -101,8 +101,8 @@ def test_fp32_kernel_options_rejected_before_device(pkg):
            (p32, dict(fp32=True, kernel="quadw")), (p32, dict(fp32=True, kernel="tile",
                                                             tile_nodes=100)),
            (p64, dict(kernel="oct")),
-           # the resident kernel is FP64 and single-state
-           (p32, dict(fp32=True, kernel="resident")), (p64, dict(kernel="resident", members=2))]
+           # the resident kernel is single-state
+           (p64, dict(kernel="resident", members=2))]
     for prob, kw in bad:
         with pytest.raises(pkg.runtime.SweHipError) as ei:
             pkg.runtime.HipContext(prob, **kw)

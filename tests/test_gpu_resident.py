@@ -87,6 +87,46 @@ def test_substeps_and_rhs_of_a_resident_context(pkg, pyoracle):
     assert np.array_equal(rhs, orc.eval_rhs(want))
 
 
+@pytest.mark.parametrize("n,exact,order,slot_block,steps", [
+    (2562, True, "host", 0, 11), (2562, True, "hilbert", 16, 11), (40962, False, "hilbert", 0, 300),
+    (40962, True, "host", 0, 20)])
+def test_fp32_bitwise_vs_lane_kernel(pkg, n, exact, order, slot_block, steps):
+    """FP32: the resident kernel keeps the lane kernel's float sums, RHS and
+    stage update (and its double update_H), bit for bit in both modes."""
+    prob = pkg.host.Problem(pkg.mesh.case_path(n, seed=1), use_rcm=True, layout="cfdl", simd=4,
+                            fp32=True)
+    print(f"case: FP32 resident N={n} exact={exact} order={order} slot_block={slot_block} "
+          f"{steps} steps", flush=True)
+    kw = dict(fp32=True, exact=exact, order=order, slot_block=slot_block)
+    with pkg.runtime.HipContext(prob, kernel="lane", **kw) as ref:
+        ref.set_state(prob.H_init)
+        ref.rk_steps(steps)
+        want = ref.get_state()
+    with pkg.runtime.HipContext(prob, kernel="resident", **kw) as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(steps // 3)
+        ctx.rk_steps(steps - steps // 3)
+        got = ctx.get_state()
+    assert np.isfinite(got).all()
+    assert np.array_equal(got, want)
+
+
+def test_fp32_auto_is_resident_and_mixes_calls(pkg):
+    prob = pkg.host.Problem(pkg.mesh.case_path(40962, seed=1), use_rcm=True, layout="cfdl",
+                            simd=4, fp32=True)
+    print("case: FP32 auto at N=40962, calls of 1, 3, 8 steps", flush=True)
+    with pkg.runtime.HipContext(prob, fp32=True, kernel="lane") as ref:
+        ref.set_state(prob.H_init)
+        ref.rk_steps(12)
+        want = ref.get_state()
+    with pkg.runtime.HipContext(prob, fp32=True) as ctx:
+        ctx.set_state(prob.H_init)
+        for k in (1, 3, 8):
+            ctx.rk_steps(k)
+        got = ctx.get_state()
+    assert np.array_equal(got, want)
+
+
 def test_too_many_nodes_per_compute_unit_is_rejected(pkg):
     prob = _prob(pkg, 163842)
     print("case: resident kernel at N=163842 (above 256 nodes per CU)", flush=True)
