@@ -931,10 +931,10 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     // quad mapping) with one-wave workgroups is the fastest stage kernel
     // measured at N = 40962, 163842 and 655362; it needs a compile-time
     // stencil width.  From kResAutoMinNodes up to 256 nodes per compute unit
-    // the resident whole-step kernel beats it (N = 40962: 6.3 against 10.2
-    // us per stage over 1000 steps); below, its barrier and per-stage
-    // memory round trip cost more than a stage launch (N = 10242: 5.3
-    // against 4.6 us).
+    // the resident whole-step kernel beats it (per stage over 1000 steps:
+    // N = 40962 4.7 against 10.2 us, N = 10242 4.05 against 4.6 us); at
+    // N = 2562 its ~20 us launch cost eats the 0.3 us per stage it saves
+    // over a 20-step call.
     if (o->kernel == SWE_HIP_KERNEL_AUTO) {
         c->kernel = m->nnbr == 31 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_QUADW;
         if (!fp32 && o->members == 1 && !o->tiled && m->nnbr == 31 && m->n >= kResAutoMinNodes
