@@ -1036,7 +1036,22 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         if (!rc) {
             std::string why;
             if (int e = resident_prepare(c, &why))
-                rc = fail(e, why);
+                rc = fail(e == SWE_HIP_EDEVICE ? SWE_HIP_EINVAL : e, why);
+        }
+        // AUTO: a mesh whose tiles do not fit the resident scheme (too many
+        // rows or neighbouring tiles, e.g. long bands of a caller's node
+        // order) or that cannot be co-resident runs the stage kernels
+        if (rc == SWE_HIP_EINVAL && c->res_auto) {
+            c->res_auto = false;
+            c->tiles.ntiles = 0;
+            if (fp32) {
+                c->kernel = (o->flags & SWE_HIP_EXACT) || c->n > 262144 ? SWE_HIP_KERNEL_LANE
+                                                                         : SWE_HIP_KERNEL_PAIR;
+                c->split32 = c->kernel == SWE_HIP_KERNEL_PAIR ? 2 : 1;
+            } else {
+                c->kernel = SWE_HIP_KERNEL_PAIR;
+            }
+            rc = 0;
         }
     }
     if (!rc && o->tiled) {
