@@ -488,7 +488,7 @@ int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err)
         r.dcoef[0] = 0.0f; r.dcoef[1] = 2.0f; r.dcoef[2] = 2.0f; r.dcoef[3] = 1.0f;
         r.nsteps = nsteps;
         r.bytes = (int)(state_count(c) * (long)sizeof(float));
-        r.spin_limit = 1u << 22;
+        r.spin_limit = c->res_spin;
         r.lds_rows = c->tiles.umax_used;
         r.stage0 = (unsigned)c->res_stages;
         void *args[] = {&m, &tl, &r};
@@ -508,7 +508,7 @@ int resident_launch(swe_hip_ctx *c, int nsteps, std::string *err)
         r.dcoef[0] = 0.0; r.dcoef[1] = 2.0; r.dcoef[2] = 2.0; r.dcoef[3] = 1.0;
         r.nsteps = nsteps;
         r.bytes = (int)(state_count(c) * (long)sizeof(double));
-        r.spin_limit = 1u << 22;
+        r.spin_limit = c->res_spin;
         r.lds_rows = 2 * c->tiles.umax_used;
         r.stage0 = (unsigned)c->res_stages;
         void *args[] = {&m, &tl, &r};

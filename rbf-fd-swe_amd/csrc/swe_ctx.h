@@ -116,6 +116,7 @@ struct swe_hip_ctx {
     long long res_stages = 0;         // stages counted since the counts were cleared
     int *d_res_nbrs = nullptr;        // [ntiles][kResMaxNbrs] tiles each tile waits for
     bool res_pending = false;         // a launch's flag is not checked yet
+    unsigned res_spin = 1u << 22;     // polls a wait may take before it gives up
     bool timing = false;
     swe_hip_timers acc{};
     long long bytes = 0;         // state buffers

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1028,6 +1029,14 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
         }
         if (!rc)
             check(hipHostMalloc((void **)&c->h_res_abort, sizeof(unsigned)), "hipHostMalloc");
+        // SWE_HIP_RESIDENT_SPIN_LIMIT: polls a wait may take (default 2^22,
+        // seconds); a small value makes waits time out, which exercises the
+        // abort path without any wait that does not end
+        if (const char *v = std::getenv("SWE_HIP_RESIDENT_SPIN_LIMIT")) {
+            const long lim = std::strtol(v, nullptr, 10);
+            if (lim >= 1 && lim <= (1L << 30))
+                c->res_spin = (unsigned)lim;
+        }
         if (!rc) {
             *c->h_res_abort = 0;
             check(hipHostGetDevicePointer((void **)&c->d_res_abort, c->h_res_abort, 0),

@@ -127,6 +127,37 @@ def test_fp32_auto_is_resident_and_mixes_calls(pkg):
     assert np.array_equal(got, want)
 
 
+def test_wait_time_out_is_reported_and_recovered(pkg, monkeypatch):
+    """A wait that exceeds its poll limit aborts the launch in every tile and
+    is reported at the next synchronising call; the next launch starts from
+    cleared counters.  The time-out is forced from the host with a limit of
+    one poll on the kernel's own bounded wait (every wave still exits)."""
+    prob = _prob(pkg, 10242)
+    print("case: resident N=10242, wait poll limit 1 (forced time-out), then a normal run",
+          flush=True)
+    with pkg.runtime.HipContext(prob, kernel="pair", order="hilbert") as ref:
+        ref.set_state(prob.H_init)
+        ref.rk_steps(30)
+        want = ref.get_state()
+    monkeypatch.setenv("SWE_HIP_RESIDENT_SPIN_LIMIT", "1")
+    with pkg.runtime.HipContext(prob, kernel="resident", order="hilbert") as ctx:
+        monkeypatch.delenv("SWE_HIP_RESIDENT_SPIN_LIMIT")
+        ctx.set_state(prob.H_init)
+        errors = 0
+        for _ in range(20):   # a time-out is likely in every launch, certain in some
+            ctx.rk_steps(30)
+            try:
+                ctx.synchronize()
+            except pkg.runtime.SweHipError as e:
+                assert e.code == -3 and "limit" in str(e)
+                errors += 1
+        assert errors > 0
+    with pkg.runtime.HipContext(prob, kernel="resident", order="hilbert") as ctx:
+        ctx.set_state(prob.H_init)
+        ctx.rk_steps(30)
+        assert np.array_equal(ctx.get_state(), want)
+
+
 def test_too_many_nodes_per_compute_unit_is_rejected(pkg):
     prob = _prob(pkg, 163842)
     print("case: resident kernel at N=163842 (above 256 nodes per CU)", flush=True)
