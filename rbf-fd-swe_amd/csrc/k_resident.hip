@@ -454,7 +454,8 @@ int resident_check(swe_hip_ctx *c, std::string *err)
     if (*c->h_res_abort) {
         *c->h_res_abort = 0;
         c->res_stages = 0;   // the counts are inconsistent: clear them next time
-        *err = "resident kernel: a grid barrier wait exceeded its limit (the state is invalid)";
+        *err = "resident kernel: a wait for neighbouring tiles exceeded its poll limit (the "
+               "state is invalid)";
         return SWE_HIP_EDEVICE;
     }
     return 0;

@@ -109,7 +109,7 @@ struct swe_hip_ctx {
     hipGraphExec_t graphs[swe::kGraphCount] = {};   // 64, 16, 4 and 1 RK steps
     // resident whole-step kernel (kernel == SWE_HIP_KERNEL_RESIDENT)
     int res_block = 0, res_lds = 0;   // threads per workgroup, dynamic LDS bytes
-    void *d_res_sync = nullptr;       // grid-barrier counters
+    void *d_res_sync = nullptr;       // abort word + per-tile stage counts
     unsigned *h_res_abort = nullptr;  // pinned: wait time-out flag of the resident launches
     unsigned *d_res_abort = nullptr;  // its device address
     bool res_auto = false;            // chosen by AUTO: short calls use the stage graphs
@@ -154,9 +154,9 @@ inline DevMesh mesh_of(const swe_hip_ctx *c)
 // returns code.
 int set_error(int code, const std::string &msg);
 
-// Resident whole-step kernel (k_resident.hip): bytes of its barrier state,
+// Resident whole-step kernel (k_resident.hip): bytes of its sync state,
 // LDS/co-residency set-up once the tiles are built, n steps (asynchronous),
-// and the check of the last launch's barrier time-out flag, to be called
+// and the check of the last launch's time-out flag, to be called
 // once the context's stream has drained.
 long resident_sync_bytes(int ntiles);
 int resident_prepare(swe_hip_ctx *c, std::string *err);
