@@ -1559,9 +1559,22 @@ int swe_hip_rk_steps(swe_hip_ctx *c, int nsteps)
     // too short to repay a launch (same arithmetic, bitwise)
     if (c->kernel == SWE_HIP_KERNEL_RESIDENT && (!c->res_auto || nsteps >= kResAutoMinSteps)) {
         std::string why;
-        if (int rc = resident_steps(c, nsteps, &why))
+        const long long before = c->res_stages;
+        int rc = resident_steps(c, nsteps, &why);
+        if (rc == SWE_HIP_EDEVICE && c->res_auto && c->res_stages == 0 && before == 0
+            && !c->res_pending) {
+            // AUTO, and the device refused the first cooperative launch
+            // (e.g. compute units reserved by another process): the stage
+            // kernels from here on
+            c->res_auto = false;
+            c->kernel = c->fp32 ? SWE_HIP_KERNEL_LANE : SWE_HIP_KERNEL_PAIR;
+            c->split32 = 1;
+            (void)hipGetLastError();
+        } else if (rc) {
             return fail(rc, why);
-        return 0;
+        } else {
+            return 0;
+        }
     }
     // every graph is captured on the first call, so no later (timed) call
     // pays for a capture
