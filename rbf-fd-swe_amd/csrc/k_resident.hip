@@ -110,13 +110,34 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc(const double *p, in
 
 constexpr int kSc1 = 16;   // cache policy: agent-coherent access
 
-__device__ __forceinline__ double2 load_sc1(__amdgpu_buffer_rsrc_t r, int off_bytes)
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// 16 bytes per lane from global memory straight into LDS, agent-coherent
+// (global_load_lds_dwordx4 ... sc1): lane l's bytes land at lds + 16 l, lds
+// being wave-uniform; the wave's vmcnt counts the load until they are there.
+__device__ __forceinline__ void dma16_sc1(const void *src, void *lds)
 {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off_bytes, 0,
-                                                                              kSc1));
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds, 16, 0,
+                                     kSc1);
 }
 
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// Copies rows [0, count) of 16 bytes from byte offsets offs[] of `base`
+// into rows[], one direct-to-LDS load per wave and 64 rows, then waits for
+// this wave's copies.
+__device__ __forceinline__ void copy_rows_sc1(const void *base, const int *offs, void *rows,
+                                              int count)
+{
+    const int lane = (int)threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const char *b = static_cast<const char *>(base);
+    char *r = static_cast<char *>(rows);
+    for (int i0 = 64 * wv; i0 < count; i0 += (int)blockDim.x) {
+        const int i = i0 + lane;
+        if (i < count)
+            dma16_sc1(b + offs[i], r + 16L * i0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+}
 
 __device__ __forceinline__ void store_sc1(__amdgpu_buffer_rsrc_t r, int off_bytes, double2 v)
 {
@@ -177,11 +198,9 @@ __global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident(DevMesh m,
                 return;
             const double *src = s == 0 ? r.H : (s == 2 ? r.Kb : r.Ka);
             double *out = s == 0 ? r.Ka : (s == 1 ? r.Kb : (s == 2 ? r.Ka : r.H));
-            const __amdgpu_buffer_rsrc_t rin = state_rsrc(src, r.bytes);
             if (!(alive = res_wait(r, tile, stage)))   // also: offsets written, rows free
                 return;
-            for (int i = tid; i < 2 * ucount; i += (int)blockDim.x)
-                rows[i] = load_sc1(rin, offs[i]);
+            copy_rows_sc1(src, offs, rows, 2 * ucount);
             __syncthreads();
             double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;   // variable 2k   x (own0, own1, oth0, oth1)
             double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;   // variable 2k+1 x (own0, own1, oth0, oth1)
@@ -263,12 +282,6 @@ struct DevResident32 {
     unsigned spin_limit;
 };
 
-__device__ __forceinline__ float4 load_sc1_f4(__amdgpu_buffer_rsrc_t r, int off_bytes)
-{
-    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off_bytes, 0,
-                                                                             kSc1));
-}
-
 __device__ __forceinline__ void store_sc1_f4(__amdgpu_buffer_rsrc_t r, int off_bytes, float4 v)
 {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off_bytes, 0, kSc1);
@@ -336,11 +349,9 @@ __global__ __launch_bounds__(64 * kResMaxWaves) void k_steps_resident32(DevMesh3
                 return;
             const float *src = s == 0 ? r.H : (s == 2 ? r.Kb : r.Ka);
             float *out = s == 0 ? r.Ka : (s == 1 ? r.Kb : (s == 2 ? r.Ka : r.H));
-            const __amdgpu_buffer_rsrc_t rin = state_rsrc32(src, r.bytes);
             if (!(alive = res_wait(sync_view, tile, stage)))   // also: offsets written, rows free
                 return;
-            for (int i = tid; i < ucount; i += (int)blockDim.x)
-                rows32[i] = load_sc1_f4(rin, offs[i]);
+            copy_rows_sc1(src, offs, rows32, ucount);
             __syncthreads();
             Sums32 S = {};
             static_for<NNBR>([&](auto jc) {
