@@ -152,14 +152,14 @@ def dominant_kernel(args):
     """Name of the stage kernel the options select (the one timed)."""
     if args.fp32:
         k = args.kernel if args.kernel != "auto" else \
-            ("resident" if 8000 <= args.nodes <= 256 * 256 else
+            ("resident" if args.nodes <= 256 * 256 else
              "lane" if args.exact or args.nodes > 262144 else "pair")
         return {"lane": "k_stage_lane32", "tile": "k_stage_tile32",
                 "resident": "k_steps_resident32"}.get(k, "k_stage_split32")
     if args.members == 1:
         if args.kernel == "auto":
             # the library's auto rule (256 nodes per CU on the 256 CUs of MI355X)
-            return "k_steps_resident" if 8000 <= args.nodes <= 256 * 256 else "k_stage_pair"
+            return "k_steps_resident" if args.nodes <= 256 * 256 else "k_stage_pair"
         return {"quadw": "k_stage_quadw", "pair": "k_stage_pair",
                 "tile": "k_stage_ptile", "resident": "k_steps_resident"}[args.kernel]
     if args.tiled:

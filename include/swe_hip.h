@@ -121,8 +121,8 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
  * the static data is reused (s may then be NULL): the static data stay
  * resident until the last context using them is destroyed.  Out-of-range
  * options fail with SWE_HIP_EINVAL before any device call. */
-#define SWE_HIP_KERNEL_AUTO 0   /* single state: RESIDENT from 8000 up to
-                                   256 nodes per compute unit (with a
+#define SWE_HIP_KERNEL_AUTO 0   /* single state: RESIDENT up to 256 nodes
+                                   per compute unit (with a
                                    Hilbert device order unless another is
                                    asked for), else PAIR with a 31-point
                                    stencil, else QUADW (FP32: see below)     */
@@ -169,7 +169,7 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
  * lanes per node, each summing every 2nd, 4th or 8th stencil slot, the
  * partial sums added across the group (fast mode only).  RESIDENT = the
  * resident whole-step kernel with one lane per node (both modes).  AUTO =
- * RESIDENT from 8000 up to 256 nodes per compute unit, else LANE in exact
+ * RESIDENT up to 256 nodes per compute unit, else LANE in exact
  * mode, and in fast mode PAIR up to 262144 nodes (one lane per node would
  * leave the chip under-filled), LANE above.  QUADW is an FP64 mapping. */
 #define SWE_HIP_ORDER_HOST 0    /* device node order = the caller's (RCM)    */

@@ -26,7 +26,7 @@ constexpr int kGraphCount = 4;         // captured step graphs per context
 // ~1100 at N = 40962, a Hilbert-ordered tile ~400).
 constexpr int kResMaxWaves = 8;
 constexpr int kResMaxRows = 2560;
-constexpr int kResAutoMinNodes = 8000;    // SWE_HIP_KERNEL_AUTO picks it from here
+constexpr int kResAutoMinNodes = 1;       // SWE_HIP_KERNEL_AUTO picks it from here
 constexpr int kResMaxNbrs = 64;           // tiles one tile exchanges rows with (one wave)
 constexpr int kResAutoMinSteps = 4;       // AUTO contexts: shorter calls replay stage graphs
 

@@ -931,11 +931,10 @@ int swe_hip_create_ex(const swe_hip_static *s, const swe_hip_options *o, swe_hip
     // auto: the pair mapping (half the address-unit work per node of the
     // quad mapping) with one-wave workgroups is the fastest stage kernel
     // measured at N = 40962, 163842 and 655362; it needs a compile-time
-    // stencil width.  From kResAutoMinNodes up to 256 nodes per compute unit
-    // the resident whole-step kernel beats it (per stage over 1000 steps:
-    // N = 40962 4.7 against 10.2 us, N = 10242 4.05 against 4.6 us); at
-    // N = 2562 its ~20 us launch cost eats the 0.3 us per stage it saves
-    // over a 20-step call.
+    // stencil width.  Up to 256 nodes per compute unit the resident
+    // whole-step kernel beats it at every size measured, its ~20 us launch
+    // included (per stage over 20-step calls: N = 642 3.4 against 4.6 us,
+    // 2562 3.6 against 4.6, 10242 3.7 against 5.1, 40962 4.8 against 10.4).
     if (o->kernel == SWE_HIP_KERNEL_AUTO) {
         c->kernel = m->nnbr == 31 ? SWE_HIP_KERNEL_PAIR : SWE_HIP_KERNEL_QUADW;
         if (!fp32 && o->members == 1 && !o->tiled && m->nnbr == 31 && m->n >= kResAutoMinNodes

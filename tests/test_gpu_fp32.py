@@ -95,7 +95,8 @@ def test_fp32_entry_points_reject_the_other_precision(pkg):
 
 
 def test_fp32_tuner_and_split_kernels_agree(pkg):
-    """swe_hip_tune_ex over the FP32 mappings returns a fast-mode kernel, and
+    """swe_hip_tune_ex over the FP32 mappings (and the resident kernel) returns a
+    fast-mode kernel, and
     every split mapping gives the same state after 10 steps (the group sums
     are formed alike in every lane; the mappings differ only in the order of
     the partial sums, so within 1e-4 of one another, FP32 rounding)."""
@@ -104,7 +105,7 @@ def test_fp32_tuner_and_split_kernels_agree(pkg):
     print("case: FP32 tuner N=10242", flush=True)
     with pkg.runtime.HipContext(prob, fp32=True, kernel="lane") as base:
         opts, ms = base.tune(members=1, steps=3)
-        assert opts["kernel"] in ("lane", "pair", "quad", "oct", "tile") and ms > 0
+        assert opts["kernel"] in ("lane", "pair", "quad", "oct", "tile", "resident") and ms > 0
         out = {}
         for k in ("lane", "pair", "quad", "oct"):
             with pkg.runtime.HipContext(prob, fp32=True, share=base, kernel=k) as ctx:
