@@ -16,8 +16,9 @@
 //
 // Cross-workgroup visibility: the L2 caches of the 8 XCDs are not coherent
 // with each other, so every state access another workgroup depends on is
-// agent-coherent (buffer loads and stores with the sc1 cache policy; the
-// published states are small and live in the last-level cache), and the
+// agent-coherent (direct-to-LDS loads and buffer stores with the sc1 cache
+// policy; the published states are small and live in the last-level
+// cache), and the
 // stage counts are agent-scope atomics written after the state stores
 // have completed.  Every wait is bounded (spin_limit polls): a workgroup
 // that exceeds it raises the abort flag, which every waiting workgroup
