@@ -161,7 +161,13 @@ int swe_hip_eval_rhs32(swe_hip_ctx *ctx, int member, const float *K, float *F);
                                    swe_hip_get_state, swe_hip_query or
                                    swe_hip_rk_steps after the stream has
                                    drained.  Drop-in substeps and eval_rhs
-                                   use PAIR (FP32: LANE).                   */
+                                   use PAIR (FP32: LANE).  A wait gives up
+                                   after 2^22 polls (seconds); the
+                                   environment variable
+                                   SWE_HIP_RESIDENT_SPIN_LIMIT, read at
+                                   context creation, sets another limit
+                                   (1 forces time-outs, for testing the
+                                   error path).                             */
 /* In SWE_HIP_FP32 contexts: LANE = one lane per node; TILE = one lane per
  * node over LDS tiles of tile_nodes (64, 128 or 256) consecutive device
  * nodes (31-point stencils; use a HILBERT node order); both sum every
