@@ -127,6 +127,33 @@ def test_fp32_auto_is_resident_and_mixes_calls(pkg):
     assert np.array_equal(got, want)
 
 
+def test_two_contexts_in_flight_at_once(pkg):
+    """Two resident contexts (two meshes, two streams) with their launches
+    queued back to back before either is waited for: both give the bits of
+    their stage-kernel runs (the runtime serialises or co-schedules the
+    cooperative launches; every wait is bounded either way)."""
+    pa, pb = _prob(pkg, 10242), _prob(pkg, 40962)
+    print("case: resident N=10242 and N=40962 launched back to back on two streams",
+          flush=True)
+    want = []
+    for p in (pa, pb):
+        with pkg.runtime.HipContext(p, kernel="pair", order="hilbert") as ref:
+            ref.set_state(p.H_init)
+            ref.rk_steps(200)
+            want.append(ref.get_state())
+    with pkg.runtime.HipContext(pa, kernel="resident", order="hilbert") as ca, \
+            pkg.runtime.HipContext(pb, kernel="resident", order="hilbert") as cb:
+        ca.set_state(pa.H_init)
+        cb.set_state(pb.H_init)
+        for _ in range(4):
+            ca.rk_steps(50)
+            cb.rk_steps(50)
+        ca.synchronize()
+        cb.synchronize()
+        assert np.array_equal(ca.get_state(), want[0])
+        assert np.array_equal(cb.get_state(), want[1])
+
+
 def test_wait_time_out_is_reported_and_recovered(pkg, monkeypatch):
     """A wait that exceeds its poll limit aborts the launch in every tile and
     is reported at the next synchronising call; the next launch starts from
