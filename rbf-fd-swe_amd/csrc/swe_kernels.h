@@ -1068,8 +1068,11 @@ __device__ __forceinline__ TmemOwn tmem_own_load(const DevStage &st, const doubl
     const long e = tmem_pair_elem(B, (lane >> 5) ? I.nB : I.nA, I.col0, lane);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        O.hv[q] = STAGE != ST_RHS ? *reinterpret_cast<const double2 *>(st.h + e + q * B)
-                                  : make_double2(0.0, 0.0);
+        // stage 0 gathers H itself: the epilogue takes H from the node's
+        // own row in the tile (tmem_compute)
+        O.hv[q] = (STAGE != ST_RHS && STAGE != ST_0)
+                      ? *reinterpret_cast<const double2 *>(st.h + e + q * B)
+                      : make_double2(0.0, 0.0);
         O.dv[q] = (STAGE != ST_RHS && STAGE != ST_0)
                       ? *reinterpret_cast<const double2 *>(st.d + e + q * B)
                       : make_double2(0.0, 0.0);
@@ -1289,10 +1292,24 @@ __device__ __forceinline__ void tmem_compute(const DevMesh &m, const DevTiles &t
     const double r1 = pair_swap(k ? F[1] : F[3]);
     const double2 Fa = k ? make_double2(r0, F[2]) : make_double2(F[0], r0);
     const double2 Fb = k ? make_double2(r1, F[3]) : make_double2(F[1], r1);
+    const long e2 = tmem_pair_elem(B, node, I.col0, lane);
+    double2 hv0 = O.hv[0], hv1 = O.hv[1];
+    if constexpr (STAGE == ST_0) {
+        // H is the gathered state: this lane's pair-layout values from the
+        // own row (variables 2k, 2k + 1 of members ml & ~1 and the next)
+        if (own != 0xffffu) {
+            const double *r = rows + own * 128 + 64 * k + (ml & ~1);
+            hv0 = *reinterpret_cast<const double2 *>(r);
+            hv1 = *reinterpret_cast<const double2 *>(r + 32);
+        } else {
+            hv0 = *reinterpret_cast<const double2 *>(st.h + e2);
+            hv1 = *reinterpret_cast<const double2 *>(st.h + e2 + B);
+            asm volatile("" : "+v"(hv0.x), "+v"(hv0.y), "+v"(hv1.x), "+v"(hv1.y));
+        }
+    }
     if (!half || I.hasB) {
-        const long e2 = tmem_pair_elem(B, node, I.col0, lane);
-        stage_update_pre_pair<STAGE, EXACT>(st, e2, Fa, O.hv[0], O.dv[0]);
-        stage_update_pre_pair<STAGE, EXACT>(st, e2 + B, Fb, O.hv[1], O.dv[1]);
+        stage_update_pre_pair<STAGE, EXACT>(st, e2, Fa, hv0, O.dv[0]);
+        stage_update_pre_pair<STAGE, EXACT>(st, e2 + B, Fb, hv1, O.dv[1]);
     }
 }
 
