@@ -497,9 +497,10 @@ def main():
         "dtype": "f32" if args.fp32 else "f64",
         "data": "synthetic (seeded icosahedral mesh + Gaussian RBF-FD weights, TC5-like state)",
         "config": {"workload": f"RBF-FD SWE RK4, N={prob.n_nodes} nodes, stencil 31, "
-                               f"{args.members} member(s), {dominant_kernel(args)} "
-                               "(the 40962-node, 1000-step single-member case timed against "
-                               "the reference CPU path)",
+                               f"{args.members} member(s), {dominant_kernel(args)}"
+                               + (" (the 40962-node, 1000-step single-member case timed "
+                                  "against the reference CPU path)"
+                                  if prob.n_nodes == 40962 and args.members == 1 else ""),
                    "nodes": prob.n_nodes, "members": args.members, "layout": args.layout,
                    "member_block": args.member_block, "kernel": args.kernel,
                    "prefetch": args.prefetch, "order": args.order, "tiled": args.tiled,
