@@ -2,8 +2,9 @@
 """Benchmark of the RBF-FD shallow-water RK4 hot path on one MI355X.
 
 Prints one JSON line.  Headline (the contract's value/roofline/cpu_baseline):
-the N=40962 icosahedral mesh, 31-point stencils, FP64, one state, RK4 with
-the fused graph-captured stage kernels; one "step" is one RK4 timestep = 4
+the N=40962 icosahedral mesh, 31-point stencils, FP64, one state, RK4 on the
+kernel the library's `auto` rule selects (the resident whole-step kernel at
+this size; --kernel names another); one "step" is one RK4 timestep = 4
 eval_RHS passes over every node; inputs are generated (seeded) in the
 reference's binary format, prepared by the C host (RCM, layout) and resident
 in HBM before the timed region.
@@ -192,6 +193,32 @@ def timed_steps(torch, ctx, k, warm=2):
     return e0.elapsed_time(e1) / k
 
 
+def stored_traffic(kernel, nodes, members):
+    """HBM bytes per launch of `kernel` on this workload as counted by
+    rocprofv3 (FETCH_SIZE, WRITE_SIZE) outside this process, with its source;
+    (None, None) when no count was taken for exactly this kernel."""
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{kernel}_n{nodes}_b{members}.json")
+    if not os.path.exists(tpath):
+        return None, None
+    with open(tpath) as fp:
+        rec = json.load(fp)
+    if rec.get("kernel") != kernel:
+        return None, None
+    return (rec.get("hbm_bytes_per_launch"),
+            f"stored rocprofv3 --pmc count of {kernel} ({os.path.relpath(tpath, ROOT)}), "
+            "not measured in this run")
+
+
+def leg_kernel(opts):
+    """Short name of the stage kernel a tuned option set runs."""
+    if opts.get("members", 1) == 1:
+        return {"tile": "k_stage_ptile", "resident": "k_steps_resident"}.get(opts["kernel"],
+                                                                             "k_stage_pair")
+    if opts["tiled"]:
+        return "k_stage_tspec"
+    return "k_stage_mpair" if opts["member_block"] == 64 else "k_stage_members"
+
+
 def describe(opts):
     if opts.get("members", 1) == 1:
         if opts["kernel"] == "tile":
@@ -223,13 +250,15 @@ def c3_leg(pkg, torch, steps=200):
     n = prob.n_nodes
     launch_us = ms * 1e3 / 4
     achieved = stage_bytes(prob.padded, 1) / (launch_us * 1e-6) / 1e9
+    traffic, traffic_source = stored_traffic(leg_kernel(opts), n, 1)
     return {"workload": f"N={n}, 1 member, {steps} RK4 steps, FP64, fused stages + HIP graphs",
             "kernel": describe(opts), "options": opts, "ms_per_step": ms,
             "gnode_rhs_per_s": n * 4 / (ms * 1e-3) / 1e9,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "bytes_per_launch": stage_bytes(prob.padded, 1),
-                         "avg_launch_us": launch_us},
+                         "avg_launch_us": launch_us, "traffic": traffic,
+                         "traffic_source": traffic_source},
             "state_finite": finite}
 
 
@@ -256,6 +285,7 @@ def ensemble_leg(pkg, torch, steps=500):
     flops_launch = float(N) * 64 * FLOP_PER_NODE_RHS
     achieved = bytes_launch / (launch_us * 1e-6) / 1e9
     tflops = flops_launch / (launch_us * 1e-6) / 1e12
+    traffic, traffic_source = stored_traffic(leg_kernel(opts), n, 64)
     return {"workload": f"N={n}, 64 members (seeded 1e-3 perturbations), {steps} RK4 steps, "
                         "Hilbert-ordered device copy, fused stages + HIP graphs",
             "kernel": describe(opts), "options": opts,
@@ -266,6 +296,7 @@ def ensemble_leg(pkg, torch, steps=500):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                          "bytes_per_launch": bytes_launch, "avg_launch_us": launch_us,
+                         "traffic": traffic, "traffic_source": traffic_source,
                          "fp64_tflops": tflops, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                          "fp64_frac": tflops / FP64_PEAK_TFLOPS},
             "state_finite": finite}
@@ -439,16 +470,8 @@ def main():
     # HBM bytes per launch are counted by rocprofv3 (FETCH_SIZE, WRITE_SIZE)
     # outside this process; the stored count is reported with its source and
     # only when it was taken for the kernel this run selects
-    traffic, traffic_source = None, None
-    tpath = os.path.join(ROOT, "profiles",
-                         f"traffic_{dominant_kernel(args)}_n{args.nodes}_b{args.members}.json")
-    if os.path.exists(tpath) and not args.fp32:
-        with open(tpath) as fp:
-            rec = json.load(fp)
-        if rec.get("kernel") == dominant_kernel(args):
-            traffic = rec.get("hbm_bytes_per_launch")
-            traffic_source = (f"stored rocprofv3 --pmc count of {rec.get('kernel')} "
-                              f"({os.path.relpath(tpath, ROOT)}), not measured in this run")
+    traffic, traffic_source = (None, None) if args.fp32 else \
+        stored_traffic(dominant_kernel(args), args.nodes, args.members)
     node_rhs = float(prob.n_nodes) * 4 * args.steps * args.members
     value = node_rhs / (gpu_ms * 1e-3) / 1e9
     ctx.close()
