@@ -418,12 +418,21 @@ void *resident_kernel(bool fp32)
 long resident_sync_bytes(int ntiles) { return (1L + ntiles) * kResLine * sizeof(unsigned); }
 
 // LDS budget of both kernel instances and a check that one workgroup per
-// tile can be resident at once (the cooperative launch needs it).
+// tile can be resident at once (the cooperative launch needs it).  The
+// budget is a property of the kernel, shared by every context: it is set to
+// what the largest tile any context may build needs (kResMaxRows rows), so
+// that a context created later never lowers it below an earlier one's use.
 int resident_prepare(swe_hip_ctx *c, std::string *err)
 {
+    const int lds_max = kResMaxRows * (c->fp32 ? (int)(sizeof(float4) + sizeof(int))
+                                               : 2 * (int)(sizeof(double2) + sizeof(int)));
+    if (c->res_lds > lds_max) {
+        *err = "resident kernel: LDS budget";
+        return SWE_HIP_EDEVICE;
+    }
     for (bool ex : {false, true}) {
         void *fn = ex ? resident_kernel<true>(c->fp32) : resident_kernel<false>(c->fp32);
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, c->res_lds)
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max)
             != hipSuccess) {
             *err = "resident kernel: LDS budget";
             return SWE_HIP_EDEVICE;
