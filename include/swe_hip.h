@@ -232,16 +232,19 @@ int swe_hip_tune_ensemble(swe_hip_ctx *base, int members, unsigned flags, int st
 /* Autotuner over the whole configuration space: times `steps` fused RK
  * steps of every candidate on base's static data and node order and writes
  * the fastest, ready for swe_hip_create_ex (share = base), to *best:
- *   members == 1: the PAIR kernel and the TILE kernel with tiles of 64, 128
- *     and 256 nodes and each row-staging schedule (prefetch 0: through
- *     registers, 4: direct-to-LDS, 8: direct-to-LDS with the first weights
- *     loaded before the tile barrier);
+ *   members == 1: the PAIR kernel, the RESIDENT kernel (31-point stencils,
+ *     at most 256 nodes per compute unit) and the TILE kernel with tiles of
+ *     64, 128 and 256 nodes and each row-staging schedule (prefetch 0:
+ *     through registers, 4: direct-to-LDS, 8: direct-to-LDS with the first
+ *     weights loaded before the tile barrier);
  *   members > 1: every member block untiled, and member block 32 on
  *     member-lane tiles of tile_nodes 8 or 6 x tile_rows 70 or 56;
  *   SWE_HIP_FP32 contexts (flags must carry SWE_HIP_FP32, members == 1):
  *     LANE, TILE with 64, 128 and 256 nodes, and in fast mode PAIR, QUAD
- *     and OCT (lanes per node).
- * No counterpart in the reference. */
+ *     and OCT (lanes per node), and RESIDENT as above.
+ * A candidate that base's mesh or node order cannot take (its create fails
+ * with SWE_HIP_EINVAL, e.g. resident tiles of a scattered node order) is
+ * passed over.  No counterpart in the reference. */
 int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
                     swe_hip_options *best, double *best_ms_per_step);
 

@@ -1205,6 +1205,9 @@ int swe_hip_tune_ex(swe_hip_ctx *base, int members, unsigned flags, int steps,
     for (size_t k = 0; k < cand.size(); ++k) {
         double per = 0.0;
         const int rc = time_options(base, cand[k], steps, &per);
+        if (rc == SWE_HIP_EINVAL)
+            continue;   // not possible on this mesh and node order (e.g. the
+                        // resident kernel's tiles of a scattered order)
         if (rc)
             return rc;
         if (per < best_t) {

@@ -191,3 +191,18 @@ def test_too_many_nodes_per_compute_unit_is_rejected(pkg):
     with pytest.raises(pkg.runtime.SweHipError) as ei:
         pkg.runtime.HipContext(prob, kernel="resident")
     assert ei.value.code == -1
+
+
+def test_scattered_order_rejects_resident_and_tuner_skips_it(pkg):
+    """In the generator's own node order (no RCM, host order) the tiles of
+    consecutive nodes are scattered over the sphere (more than 64
+    neighbouring tiles): a resident context is refused with SWE_HIP_EINVAL,
+    and the autotuner passes over that candidate instead of failing."""
+    prob = _prob(pkg, 10242, rcm=False)
+    print("case: resident kernel and tuner on a scattered host order, N=10242", flush=True)
+    with pytest.raises(pkg.runtime.SweHipError) as ei:
+        pkg.runtime.HipContext(prob, kernel="resident", order="host")
+    assert ei.value.code == -1
+    with pkg.runtime.HipContext(prob, kernel="pair", order="host") as base:
+        opts, ms = base.tune(1, steps=2)
+    assert opts["kernel"] != "resident" and ms > 0
