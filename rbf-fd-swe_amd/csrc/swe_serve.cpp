@@ -205,14 +205,14 @@ int new_batch(swe_srv *s, int mesh_id, int *batch_id)
                 evict(s, bi);
         }
     }
-    if (s->hbm_in_use + need() > s->cfg.hbm_cap_bytes) {
-        bool will_free = false;   // a live member or a chunk in flight
+    auto will_free = [&] {   // a live member or a chunk in flight
         for (const Batch &b : s->batches)
-            will_free = will_free || (b.ctx && (b.busy || b.active > 0));
-        if (!will_free)
-            return SWE_HIP_ENOMEM;   // the batch can never fit the budget
-        return 1;   // over budget: not an error, the member waits
-    }
+            if (b.ctx && (b.busy || b.active > 0))
+                return true;
+        return false;
+    };
+    if (s->hbm_in_use + need() > s->cfg.hbm_cap_bytes)
+        return will_free() ? 1 : SWE_HIP_ENOMEM;   // waits, or can never fit the budget
     swe_hip_ctx *owner = nullptr;
     for (int bi : m.batches)
         if (s->batches[bi].ctx) {
@@ -224,8 +224,8 @@ int new_batch(swe_srv *s, int mesh_id, int *batch_id)
                       m.tiled ? 1 : 0, 0, 0, 0};
     swe_hip_ctx *ctx = nullptr;
     int rc = swe_hip_create_ex(owner ? nullptr : &m.desc, &o, &ctx);
-    if (rc == SWE_HIP_ENOMEM)
-        return 1;
+    if (rc == SWE_HIP_ENOMEM)   // the device itself is full: wait while work
+        return will_free() ? 1 : SWE_HIP_ENOMEM;   // is in flight, else fail
     if (rc)
         return rc;
     Batch b;
